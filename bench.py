@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: wet-cell updates per second of the KP step on one MI355X.
+
+Workload (N=1): rain_basin 2048 x 2048 cells, dx = 10 m, diamond-square DEM
+0-60 m with a carved channel, n = 0.035, triangular hyetograph 0 -> 80 mm/hr
+over 1 h, Green-Ampt K lognormal (median 5 mm/hr, sigma_ln 0.8), psi 0.11,
+dtheta 0.3, tile acceleration on, inputs generated with fixed seeds and
+resident in HBM before the timed region.
+
+A step is one full Heun time step (both stages, forcing, tile bookkeeping,
+dt).  A wet-cell update (WCU) is one interior cell of an active 16 x 14 tile
+advanced through one step.  value = WCU over the K timed steps / wall time of
+those steps, bracketed by stream synchronisations.
+
+roofline: the dominant kernel (the flux pass) timed with HIP events on the
+handle's stream, algorithmic bytes per launch = WCU of the launch x the
+per-cell bytes of its contract, against 8 TB/s.
+cpu_baseline: the FP32 OpenMP transcription of the same per-cell update
+(oracle/liboracle32.so) on the same workload, a bounded number of steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+# bytes per wet cell of one flux launch (rain_basin flags): reads h, hu, hv,
+# BC, BX, BY, n, K, F (9 x 4) and writes dh, dhu, dhv, G, dF (5 x 4)
+FLUX_BYTES_PER_CELL = 56
+# algorithmic bytes per WCU of a whole step for this workload (state once per
+# stage, one bed field, Manning/K maps, Green-Ampt F)
+STEP_BYTES_PER_WCU = 96
+
+
+def build_workload(n_cells):
+    from rift_amd import kp, synth
+    cfg = synth.rain_basin(n_cells, seed=2)
+    flags = kp.TILE_ACCEL | kp.RAIN_AVG
+    g = kp.grid_from_rasters(cfg["dem"], cfg["dx"], n_const=cfg["n_const"], K=cfg["K"],
+                             psi=cfg["psi"], dtheta=cfg["dtheta"], flags=flags)
+    t, v = synth.series_hours_to_seconds(cfg["hyeto_hours"], cfg["hyeto_mmhr"])
+    g.set_series(kp.SER_HYETO, t, v)
+    return g, cfg
+
+
+def cpu_baseline(cfg, g, budget_s=15.0, max_steps=60):
+    """Time the FP32 OpenMP transcription on the host cores, continuing from
+    the GPU's state (h, hu, hv, F, tile flags, t, dt copied over like a
+    hotstart), so both legs time the same step window of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = O.Oracle(cfg["dem"], cfg["dx"], n_const=cfg["n_const"], K=cfg["K"], psi=cfg["psi"],
+                 dtheta=cfg["dtheta"], flags=O.TILE_ACCEL | O.RAIN_AVG, fp32=True)
+    o.set_series_hours(O.SER_HYETO, cfg["hyeto_hours"], cfg["hyeto_mmhr"])
+    bc = o.field("bc")
+    o.set_field("w", g.field("depth") + bc)
+    o.set_field("hu", g.field("hu"))
+    o.set_field("hv", g.field("hv"))
+    o.set_field("F", g.field("F"))
+    wet, act, _ = g.tiles()
+    o.set_tiles(wet, np.where(act > 0, np.arange(g.ntiles), -np.arange(g.ntiles)))
+    t, dt = g.time
+    o.set_time(t, dt)
+    o.step(1)                       # warm-up (also restores the forcing state)
+    cells = 0
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps and time.perf_counter() - t0 < budget_s:
+        o.step()
+        wet, act, n_act = o.tiles()
+        cells += _active_interior_cells(o.nx, o.ny, o.nbx, act)
+        steps += 1
+    el = time.perf_counter() - t0
+    n = cfg["dem"].shape[0] - 1
+    return {"value": cells / el, "unit": "WCU/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of rain_basin {n}x{n} continuing from the GPU state after the "
+                      f"GPU's timed window (1 untimed step), FP32 OpenMP transcription of the same "
+                      f"per-cell update (oracle/liboracle32.so), {el:.1f} s, {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=160)
+    ap.add_argument("--cells", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the solver runs one domain per GPU; see DESIGN.md for how it would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    g, cfg = build_workload(args.cells)
+    print(f"[bench] workload ready: {g.nx}x{g.ny} internal, {g.ntiles} tiles", file=sys.stderr,
+          flush=True)
+    g.step(args.warmup)
+    g.sync()
+    _, wcu0 = g.wet_cells()
+    wet, act, n_act = g.tiles()
+    print(f"[bench] warm-up done: {n_act}/{g.ntiles} tiles active", file=sys.stderr, flush=True)
+
+    # timed region: K steps enqueued back to back, stream synchronised on both sides
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    per_step, wcu1 = g.wet_cells()
+    wcu = wcu1 - wcu0
+
+    # kernel durations: a further window with every launch bracketed by HIP
+    # events on the handle's stream (kept out of the wall-clock window above)
+    kp_steps = min(args.steps, 100)
+    g.profile(True)
+    _, wk0 = g.wet_cells()
+    g.step(kp_steps)
+    g.sync()
+    _, wk1 = g.wet_cells()
+    n1, ms1 = g.kernel_stats("flux1")
+    n2, ms2 = g.kernel_stats("stage2_flux")
+    ni, msi = g.kernel_stats("integrate")
+    g.profile(False)
+    t_sim, dt = g.time
+    mb = g.mass_balance()
+    wet, act, n_act = g.tiles()
+
+    value = wcu / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    flux_avg_s = (ms1 + ms2) / max(n1 + n2, 1) * 1e-3
+    flux_bytes = ((wk1 - wk0) / kp_steps) * FLUX_BYTES_PER_CELL
+    achieved = flux_bytes / flux_avg_s if flux_avg_s > 0 else 0.0
+    line = {
+        "metric": "wet-cell updates/s",
+        "value": value,
+        "unit": "WCU/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"rain_basin {args.cells}x{args.cells} dx=10m, rain+Green-Ampt+Manning, "
+                               "tile acceleration on", "cells": args.cells, "tiles": g.ntiles,
+                   "active_tiles": int(n_act), "wet_cells_per_step": wcu / args.steps,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_flux", "bytes_per_launch": flux_bytes,
+                     "avg_launch_ms": flux_avg_s * 1e3},
+        "step_roofline": {"bytes_per_wcu": STEP_BYTES_PER_WCU,
+                          "achieved_GBps": value * STEP_BYTES_PER_WCU / 1e9,
+                          "frac": value * STEP_BYTES_PER_WCU / HBM_PEAK},
+        "kernel_ms_per_step": {"flux": (ms1 + ms2) / kp_steps, "integrate": msi / kp_steps},
+        "sim_time_s": t_sim,
+        "mass_balance_m3": {"stored": mb[0], "infiltrated": mb[1], "rain_in": mb[2]},
+    }
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(cfg, g)
+    else:
+        line["cpu_baseline"] = None
+    g.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

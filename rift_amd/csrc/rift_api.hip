@@ -1,0 +1,750 @@
+// rift_api.hip -- the C ABI of include/rift_kp.h over the gfx950 kernels.
+//
+// A handle owns one stream and every device buffer of one domain.  Host
+// arrays cross the boundary in the reference's double layout and are
+// converted to the FP32 device layout described in kp_kernels.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/rift_kp.h"
+#include "kp_device.hpp"
+#include "kp_kernels.hpp"
+
+using namespace rift;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(RIFT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kProfClasses = 6;
+
+}  // namespace
+
+struct rift_grid {
+    rift_desc desc{};
+    Dev d{};
+    hipStream_t stream = nullptr;
+    int R = 0;                 // padded rows
+    size_t plane = 0;          // P*R floats
+    std::vector<void*> allocs;
+    std::vector<double> bc_ref;   // reference BC (InitGrid_k rule), host copy
+    std::vector<double> n_host;
+    double* dsum = nullptr;       // device scratch for FP64 sums
+    int grid_tiles = 0;
+    bool static_loaded = false, inited = false;
+    bool prof = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+float* alloc_plane(rift_grid* g)
+{
+    void* p = nullptr;
+    if (hipMalloc(&p, g->plane * sizeof(float)) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, g->plane * sizeof(float)) != hipSuccess) return nullptr;
+    g->allocs.push_back(p);
+    return static_cast<float*>(p);
+}
+
+template <class T>
+T* alloc_bytes(rift_grid* g, size_t n)
+{
+    void* p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T) + 64) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, n * sizeof(T) + 64) != hipSuccess) return nullptr;
+    g->allocs.push_back(p);
+    return static_cast<T*>(p);
+}
+
+hipEvent_t take_event(rift_grid* g)
+{
+    if (!g->event_pool.empty()) {
+        hipEvent_t e = g->event_pool.back();
+        g->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+struct ProfScope {
+    rift_grid* g;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(rift_grid* g_, int c) : g(g_), cls(c)
+    {
+        if (g->prof) {
+            a = take_event(g);
+            b = take_event(g);
+            (void)hipEventRecord(a, g->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (g->prof) {
+            (void)hipEventRecord(b, g->stream);
+            g->prof_events.push_back({cls, {a, b}});
+        }
+    }
+};
+
+// Host -> device field conversion into the padded layout.
+int put_field(rift_grid* g, float* dst, const double* src, int cols, int rows, int ld)
+{
+    std::vector<float> tmp(g->plane, 0.f);
+    for (int j = 0; j < rows; ++j)
+        for (int i = 0; i < cols; ++i) tmp[(size_t)j * g->d.P + i] = (float)src[(size_t)j * ld + i];
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), g->plane * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int get_plane(rift_grid* g, const float* src, std::vector<float>& tmp)
+{
+    tmp.resize(g->plane);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, g->plane * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(RIFT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return RIFT_OK;
+}
+
+float* map_plane(rift_grid* g, int which, int plane)
+{
+    if (which < 0 || which > 2 || plane < 0 || plane > 2) return nullptr;
+    return g->d.map[which][plane];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rift_last_error(void) { return g_last_error.c_str(); }
+
+int rift_create(const rift_desc* desc, rift_grid** out)
+{
+    if (!desc || !out) return fail(RIFT_E_ARG, "rift_create: null argument");
+    if (desc->nx < 6 || desc->ny < 6) return fail(RIFT_E_ARG, "rift_create: nx, ny must be >= 6");
+    if (!(desc->dx > 0) || !(desc->dy > 0)) return fail(RIFT_E_ARG, "rift_create: dx, dy must be > 0");
+    if (!(desc->courant > 0) || !(desc->courant < 1))
+        return fail(RIFT_E_ARG, "rift_create: courant must lie in (0, 1)");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (desc->device < 0 || desc->device >= ndev)
+        return fail(RIFT_E_ARG, "rift_create: no such HIP device");
+    HIPCHK(hipSetDevice(desc->device));
+
+    rift_grid* g = new rift_grid();
+    g->desc = *desc;
+    Dev& d = g->d;
+    d.nx = desc->nx;
+    d.ny = desc->ny;
+    d.nbx = (int)std::ceil((double)d.nx / (double)kTileW);   // grid.cu:336-337
+    d.nby = (int)std::ceil((double)d.ny / (double)kTileH);
+    d.ntiles = d.nbx * d.nby;
+    d.P = ((kTileW * d.nbx + 4 + 63) / 64) * 64;
+    g->R = kTileH * d.nby + 4;
+    g->plane = (size_t)d.P * (size_t)g->R;
+    d.dx = desc->dx;
+    d.dy = desc->dy;
+    d.inv_dx = (float)(1.0 / desc->dx);
+    d.inv_dy = (float)(1.0 / desc->dy);
+    d.kappa = (float)desc->kappa;
+    d.kappa_d = desc->kappa;
+    d.courant = desc->courant;
+    d.psi_dtheta = (float)(desc->psi * desc->dtheta);
+    d.nodata = (float)desc->nodata;
+    d.flags = desc->flags;
+    d.si0 = d.sj0 = 1;
+    d.si1 = d.sj1 = 0;
+    d.nsrc = 0;
+
+    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(RIFT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    const unsigned f = desc->flags;
+    bool ok = true;
+    auto P = [&](float*& p) { p = alloc_plane(g); ok = ok && p; };
+    P(d.h); P(d.hu); P(d.hv); P(d.h0); P(d.hu0); P(d.hv0);
+    P(d.dh); P(d.dhu); P(d.dhv); P(d.G); P(d.n); P(d.BC); P(d.BX); P(d.BY);
+    if (f & RIFT_INFILTRATION) { P(d.F); P(d.F0); P(d.dF); P(d.K); }
+    if (f & RIFT_SAVE_MAX) { P(d.hmax); P(d.qmax); P(d.tpeak); P(d.tdry); }
+    if (f & RIFT_SAVE_ARRIVAL) P(d.twet);
+    if (f & RIFT_RAIN_GRID) for (int k = 0; k < 3; ++k) P(d.map[RIFT_MAP_RAIN][k]);
+    if (f & RIFT_DRAIN_GRID) for (int k = 0; k < 3; ++k) P(d.map[RIFT_MAP_DRAIN][k]);
+    if (f & RIFT_SURGE_GRID) for (int k = 0; k < 3; ++k) P(d.map[RIFT_MAP_SURGE][k]);
+    d.wet = alloc_bytes<unsigned char>(g, d.ntiles);
+    d.act = alloc_bytes<unsigned char>(g, d.ntiles);
+    d.list = alloc_bytes<int>(g, d.ntiles);
+    d.mx = alloc_bytes<float>(g, d.ntiles);
+    d.my = alloc_bytes<float>(g, d.ntiles);
+    d.sc = alloc_bytes<Scalars>(g, 1);
+    d.ser = alloc_bytes<SeriesTable>(g, 3);
+    g->dsum = alloc_bytes<double>(g, 4);
+    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.sc && d.ser && g->dsum;
+    if (!ok) {
+        rift_destroy(g);
+        return fail(RIFT_E_HIP, "rift_create: device allocation failed");
+    }
+    g->grid_tiles = std::min(d.ntiles, 256 * 8);
+    *out = g;
+    return RIFT_OK;
+}
+
+int rift_destroy(rift_grid* g)
+{
+    if (!g) return RIFT_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (auto& pe : g->prof_events) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    for (hipEvent_t e : g->event_pool) (void)hipEventDestroy(e);
+    for (void* p : g->allocs) (void)hipFree(p);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return RIFT_OK;
+}
+
+int rift_upload_static(rift_grid* g, const double* BX, const double* BY, const double* n,
+                       const double* K, const double* h0)
+{
+    if (!g || !BX || !BY) return fail(RIFT_E_ARG, "rift_upload_static: null argument");
+    Dev& d = g->d;
+    const int nx = d.nx, ny = d.ny;
+    int rc;
+    if ((rc = put_field(g, d.BX, BX, nx + 1, ny, nx + 1))) return rc;
+    if ((rc = put_field(g, d.BY, BY, nx, ny + 1, nx))) return rc;
+    // InitGrid_k (grid.cu:531-537): BC = 0.25f*(BE+BW+BN+BS) on i, j >= 2.
+    g->bc_ref.assign((size_t)nx * ny, 0.0);
+    for (int j = 2; j < ny; ++j) {
+        for (int i = 2; i < nx; ++i) {
+            double BE = BX[(size_t)j * (nx + 1) + i + 1], BW = BX[(size_t)j * (nx + 1) + i];
+            double BN = BY[(size_t)(j + 1) * nx + i], BS = BY[(size_t)j * nx + i];
+            g->bc_ref[(size_t)j * nx + i] = 0.25f * (BE + BW + BN + BS);
+        }
+    }
+    // Device BC: interior as above; every other cell takes the BC of the
+    // nearest interior cell, the cell its ghost value is copied from.
+    std::vector<double> bc(g->plane, 0.0);
+    for (int j = 0; j < g->R; ++j) {
+        int jc = std::min(std::max(j, 2), ny - 3);
+        for (int i = 0; i < d.P; ++i) {
+            int ic = std::min(std::max(i, 2), nx - 3);
+            bc[(size_t)j * d.P + i] = g->bc_ref[(size_t)jc * nx + ic];
+        }
+    }
+    if ((rc = put_field(g, d.BC, bc.data(), d.P, g->R, d.P))) return rc;
+    std::vector<double> zero((size_t)nx * ny, 0.0);
+    g->n_host.assign(n ? n : zero.data(), (n ? n : zero.data()) + (size_t)nx * ny);
+    if ((rc = put_field(g, d.n, g->n_host.data(), nx, ny, nx))) return rc;
+    if (d.K) {
+        if (!K) return fail(RIFT_E_ARG, "rift_upload_static: K required with infiltration");
+        if ((rc = put_field(g, d.K, K, nx, ny, nx))) return rc;
+    }
+    // the initial depth is held in h0 until rift_init
+    if ((rc = put_field(g, d.h0, h0 ? h0 : zero.data(), nx, ny, nx))) return rc;
+    g->static_loaded = true;
+    return RIFT_OK;
+}
+
+int rift_init(rift_grid* g, double t0)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_init: null handle");
+    if (!g->static_loaded) return fail(RIFT_E_STATE, "rift_init: call rift_upload_static first");
+    Dev& d = g->d;
+    const size_t bytes = g->plane * sizeof(float);
+    // w = BC + h0 on the interior (InitGrid_k, grid.cu:539-547); h0 = 0 elsewhere
+    if (d.flags & RIFT_H_INIT) HIPCHK(hipMemcpyAsync(d.h, d.h0, bytes, hipMemcpyDeviceToDevice, g->stream));
+    else HIPCHK(hipMemsetAsync(d.h, 0, bytes, g->stream));
+    HIPCHK(hipMemcpyAsync(d.h0, d.h, bytes, hipMemcpyDeviceToDevice, g->stream));
+    float* zeros[] = {d.hu, d.hv, d.hu0, d.hv0, d.dh, d.dhu, d.dhv, d.G,
+                      d.F, d.F0, d.dF, d.hmax, d.qmax, d.tpeak, d.tdry};
+    for (float* p : zeros)
+        if (p) HIPCHK(hipMemsetAsync(p, 0, bytes, g->stream));
+    if (d.twet) {
+        std::vector<float> tw(g->plane, 0.f);
+        for (int j = 2; j < d.ny; ++j)
+            for (int i = 2; i < d.nx; ++i) tw[(size_t)j * d.P + i] = -9999.f;
+        HIPCHK(hipMemcpyAsync(d.twet, tw.data(), bytes, hipMemcpyHostToDevice, g->stream));
+        HIPCHK(hipStreamSynchronize(g->stream));
+    }
+    // tiles whose origin cell lies inside nx x ny start wet and active
+    std::vector<unsigned char> wet(d.ntiles, 0);
+    for (int by = 0; by < d.nby; ++by)
+        for (int bx = 0; bx < d.nbx; ++bx)
+            if (bx * kTileW + 2 < d.nx && by * kTileH + 2 < d.ny) wet[by * d.nbx + bx] = 1;
+    HIPCHK(hipMemcpyAsync(d.wet, wet.data(), d.ntiles, hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(d.act, wet.data(), d.ntiles, hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemsetAsync(d.mx, 0, d.ntiles * sizeof(float), g->stream));
+    HIPCHK(hipMemsetAsync(d.my, 0, d.ntiles * sizeof(float), g->stream));
+    Scalars sc{};
+    HIPCHK(hipMemcpyAsync(&sc, d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    int ext = sc.hydro_external;
+    double hv = sc.hydro_value;
+    sc = Scalars{};
+    sc.t = t0;
+    sc.dt = (double)0.0000001f;   // simulator.cu:35
+    sc.hydro_external = ext;
+    sc.hydro_value = hv;
+    HIPCHK(hipMemcpyAsync(d.sc, &sc, sizeof(Scalars), hipMemcpyHostToDevice, g->stream));
+    launch_boundaries(d, g->stream);
+    int rc = check_launch("rift_init");
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    g->inited = true;
+    return RIFT_OK;
+}
+
+int rift_set_series(rift_grid* g, int which, const float* time_s, const float* value, int n)
+{
+    if (!g || which < 0 || which > 2 || n < 1 || !time_s || !value)
+        return fail(RIFT_E_ARG, "rift_set_series: bad argument");
+    if (n > kMaxSeries) return fail(RIFT_E_ARG, "rift_set_series: too many points");
+    std::vector<SeriesTable> tab(1);
+    tab[0].n = n;
+    std::memcpy(tab[0].time, time_s, n * sizeof(float));
+    std::memcpy(tab[0].rate, value, n * sizeof(float));
+    HIPCHK(hipMemcpyAsync(g->d.ser + which, tab.data(), sizeof(SeriesTable), hipMemcpyHostToDevice,
+                          g->stream));
+    if (which == RIFT_SER_HYDROGRAPH) {
+        int zero = 0;
+        HIPCHK(hipMemcpyAsync(&g->d.sc->hydro_external, &zero, sizeof(int), hipMemcpyHostToDevice,
+                              g->stream));
+    }
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_set_source_cells(rift_grid* g, const int* idx, long n)
+{
+    if (!g || (n > 0 && !idx)) return fail(RIFT_E_ARG, "rift_set_source_cells: bad argument");
+    Dev& d = g->d;
+    if (n == 0) {
+        d.si0 = d.sj0 = 1;
+        d.si1 = d.sj1 = 0;
+        d.nsrc = 0;
+        return RIFT_OK;
+    }
+    int i0 = 1 << 30, i1 = -1, j0 = 1 << 30, j1 = -1;
+    std::vector<long> seen;
+    for (long k = 0; k < n; ++k) {
+        int i = idx[k] % d.nx, j = idx[k] / d.nx;
+        if (idx[k] < 0 || j >= d.ny)
+            return fail(RIFT_E_UNSUPPORTED, "rift_set_source_cells: source cell outside the domain");
+        i0 = std::min(i0, i); i1 = std::max(i1, i);
+        j0 = std::min(j0, j); j1 = std::max(j1, j);
+        seen.push_back(idx[k]);
+    }
+    std::sort(seen.begin(), seen.end());
+    seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+    if ((long)seen.size() != n || (long)(i1 - i0 + 1) * (j1 - j0 + 1) != n)
+        return fail(RIFT_E_UNSUPPORTED, "rift_set_source_cells: cells must form a rectangle");
+    d.si0 = i0; d.si1 = i1; d.sj0 = j0; d.sj1 = j1;
+    d.nsrc = n;
+    return RIFT_OK;
+}
+
+int rift_set_source_value(rift_grid* g, double value)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_set_source_value: null handle");
+    struct { int ext; int pad; double v; } s{1, 0, value};
+    HIPCHK(hipMemcpyAsync(&g->d.sc->hydro_external, &s.ext, sizeof(int), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(&g->d.sc->hydro_value, &s.v, sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_set_map(rift_grid* g, int which, int plane, const double* host)
+{
+    if (!g || !host) return fail(RIFT_E_ARG, "rift_set_map: null argument");
+    float* p = map_plane(g, which, plane);
+    if (!p) return fail(RIFT_E_STATE, "rift_set_map: map not enabled by the flags");
+    return put_field(g, p, host, g->d.nx, g->d.ny, g->d.nx);
+}
+
+int rift_interp_map(rift_grid* g, int which, double factor)
+{
+    if (!g || !map_plane(g, which, 0)) return fail(RIFT_E_STATE, "rift_interp_map: map not enabled");
+    launch_interp_map(g->d, which, (float)factor, g->stream);
+    return check_launch("rift_interp_map");
+}
+
+int rift_copy_map(rift_grid* g, int which, int dst, int src)
+{
+    float* a = g ? map_plane(g, which, dst) : nullptr;
+    float* b = g ? map_plane(g, which, src) : nullptr;
+    if (!a || !b) return fail(RIFT_E_STATE, "rift_copy_map: map not enabled");
+    HIPCHK(hipMemcpyAsync(a, b, g->plane * sizeof(float), hipMemcpyDeviceToDevice, g->stream));
+    return RIFT_OK;
+}
+
+#define NEED_INIT(g)                                                        \
+    do {                                                                    \
+        if (!(g)) return fail(RIFT_E_ARG, "null handle");                   \
+        if (!(g)->inited) return fail(RIFT_E_STATE, "rift_init not called"); \
+    } while (0)
+
+int rift_update_source(rift_grid* g)
+{
+    NEED_INIT(g);
+    launch_begin_step(g->d, g->stream);
+    return check_launch("begin_step");
+}
+
+int rift_grow(rift_grid* g)
+{
+    NEED_INIT(g);
+    ProfScope ps(g, 4);
+    if (g->d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(g->d, g->stream);
+    launch_grow(g->d, g->stream);
+    return check_launch("grow");
+}
+
+int rift_compute_fluxes(rift_grid* g)
+{
+    NEED_INIT(g);
+    ProfScope ps(g, 0);
+    launch_flux(g->d, 1, g->stream, g->grid_tiles);
+    return check_launch("flux");
+}
+
+int rift_compute_timestep(rift_grid* g)
+{
+    NEED_INIT(g);
+    ProfScope ps(g, 5);
+    launch_timestep(g->d, g->stream);
+    return check_launch("timestep");
+}
+
+int rift_integrate_1(rift_grid* g)
+{
+    NEED_INIT(g);
+    ProfScope ps(g, 2);
+    launch_integrate1(g->d, g->stream, g->grid_tiles);
+    return check_launch("integrate1");
+}
+
+int rift_integrate_2(rift_grid* g)
+{
+    NEED_INIT(g);
+    ProfScope ps(g, 2);
+    launch_integrate2(g->d, g->stream, g->grid_tiles);
+    return check_launch("integrate2");
+}
+
+int rift_apply_boundaries(rift_grid* g)
+{
+    NEED_INIT(g);
+    ProfScope ps(g, 3);
+    launch_boundaries(g->d, g->stream);
+    return check_launch("boundaries");
+}
+
+int rift_step(rift_grid* g, int nsteps)
+{
+    NEED_INIT(g);
+    if (nsteps < 0) return fail(RIFT_E_ARG, "rift_step: negative step count");
+    const bool euler = g->d.flags & RIFT_EULER;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc;
+        if (g->d.flags & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG))
+            launch_begin_step(g->d, g->stream);
+        if ((rc = rift_grow(g))) return rc;
+        if ((rc = rift_compute_fluxes(g))) return rc;
+        if ((rc = rift_compute_timestep(g))) return rc;
+        if ((rc = rift_integrate_1(g))) return rc;
+        if ((rc = rift_apply_boundaries(g))) return rc;
+        if (!euler) {
+            {
+                ProfScope ps(g, 1);
+                launch_flux(g->d, 2, g->stream, g->grid_tiles);
+                if ((rc = check_launch("flux2"))) return rc;
+            }
+            if ((rc = rift_integrate_2(g))) return rc;
+            if ((rc = rift_apply_boundaries(g))) return rc;
+        }
+    }
+    return RIFT_OK;
+}
+
+int rift_sync(rift_grid* g)
+{
+    if (!g) return fail(RIFT_E_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_get_time(rift_grid* g, double* t, double* dt)
+{
+    if (!g || !t || !dt) return fail(RIFT_E_ARG, "rift_get_time: null argument");
+    double v[2];
+    HIPCHK(hipMemcpyAsync(v, g->d.sc, 2 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    *t = v[0];
+    *dt = v[1];
+    return RIFT_OK;
+}
+
+int rift_set_time(rift_grid* g, double t, double dt)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_set_time: null handle");
+    double v[2] = {t, dt};
+    HIPCHK(hipMemcpyAsync(g->d.sc, v, 2 * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+static float* field_plane(rift_grid* g, int field)
+{
+    Dev& d = g->d;
+    switch (field) {
+    case RIFT_HU: return d.hu;
+    case RIFT_HV: return d.hv;
+    case RIFT_DEPTH: return d.h;
+    case RIFT_N: return d.n;
+    case RIFT_K: return d.K;
+    case RIFT_F: return d.F;
+    case RIFT_DW: return d.dh;
+    case RIFT_DHU: return d.dhu;
+    case RIFT_DHV: return d.dhv;
+    case RIFT_G: return d.G;
+    case RIFT_DF: return d.dF;
+    case RIFT_HMAX: return d.hmax;
+    case RIFT_QMAX: return d.qmax;
+    case RIFT_TWET: return d.twet;
+    case RIFT_TPEAK: return d.tpeak;
+    case RIFT_TDRY: return d.tdry;
+    case RIFT_SURGE: return d.map[RIFT_MAP_SURGE][0];
+    case RIFT_RAINMAP: return d.map[RIFT_MAP_RAIN][0];
+    case RIFT_DRAINMAP: return d.map[RIFT_MAP_DRAIN][0];
+    default: return nullptr;
+    }
+}
+
+int rift_download(rift_grid* g, int field, double* host)
+{
+    if (!g || !host) return fail(RIFT_E_ARG, "rift_download: null argument");
+    Dev& d = g->d;
+    const int nx = d.nx, ny = d.ny;
+    std::vector<float> a, b, c;
+    int rc;
+    auto at = [&](const std::vector<float>& v, int i, int j) { return v[(size_t)j * d.P + i]; };
+    if (field == RIFT_BC) {
+        std::copy(g->bc_ref.begin(), g->bc_ref.end(), host);
+        return RIFT_OK;
+    }
+    if (field == RIFT_W) {
+        if ((rc = get_plane(g, d.h, a)) || (rc = get_plane(g, d.BC, b))) return rc;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                host[(size_t)j * nx + i] = (double)at(a, i, j) + (double)at(b, i, j);
+        return RIFT_OK;
+    }
+    if (field == RIFT_Q) {
+        if ((rc = get_plane(g, d.h, a)) || (rc = get_plane(g, d.hu, b)) || (rc = get_plane(g, d.hv, c)))
+            return rc;
+        const float kap = d.kappa;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                float h = at(a, i, j), f = 2.f * h / (h * h + std::max(h * h, kap));
+                float u = f * at(b, i, j), v = f * at(c, i, j);
+                host[(size_t)j * nx + i] = (double)(h * std::sqrt(v * v + u * u));
+            }
+        return RIFT_OK;
+    }
+    float* p = field_plane(g, field);
+    if (!p) return fail(RIFT_E_ARG, "rift_download: field not available with these flags");
+    if ((rc = get_plane(g, p, a))) return rc;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) host[(size_t)j * nx + i] = (double)at(a, i, j);
+    return RIFT_OK;
+}
+
+int rift_upload(rift_grid* g, int field, const double* host)
+{
+    if (!g || !host) return fail(RIFT_E_ARG, "rift_upload: null argument");
+    Dev& d = g->d;
+    const int nx = d.nx, ny = d.ny;
+    if (field == RIFT_W) {
+        // depth = w - BC with the device's cell beds
+        std::vector<float> bc;
+        int rc = get_plane(g, d.BC, bc);
+        if (rc) return rc;
+        std::vector<double> hh((size_t)nx * ny);
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                hh[(size_t)j * nx + i] = host[(size_t)j * nx + i] - (double)bc[(size_t)j * d.P + i];
+        return put_field(g, d.h, hh.data(), nx, ny, nx);
+    }
+    float* p = field_plane(g, field);
+    if (!p || field == RIFT_Q) return fail(RIFT_E_ARG, "rift_upload: field not available");
+    return put_field(g, p, host, nx, ny, nx);
+}
+
+int rift_get_tiles(rift_grid* g, unsigned char* wet, unsigned char* active, int* n_active)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_get_tiles: null handle");
+    if (wet) HIPCHK(hipMemcpyAsync(wet, g->d.wet, g->d.ntiles, hipMemcpyDeviceToHost, g->stream));
+    if (active) HIPCHK(hipMemcpyAsync(active, g->d.act, g->d.ntiles, hipMemcpyDeviceToHost, g->stream));
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, &g->d.sc->n_active, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    if (n_active) *n_active = n;
+    return RIFT_OK;
+}
+
+int rift_set_tiles(rift_grid* g, const unsigned char* wet, const unsigned char* active)
+{
+    if (!g || !wet || !active) return fail(RIFT_E_ARG, "rift_set_tiles: null argument");
+    HIPCHK(hipMemcpyAsync(g->d.wet, wet, g->d.ntiles, hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(g->d.act, active, g->d.ntiles, hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_get_tile_speeds(rift_grid* g, float* mx, float* my)
+{
+    if (!g || !mx || !my) return fail(RIFT_E_ARG, "rift_get_tile_speeds: null argument");
+    HIPCHK(hipMemcpyAsync(mx, g->d.mx, g->d.ntiles * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipMemcpyAsync(my, g->d.my, g->d.ntiles * sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_get_tile_dims(rift_grid* g, int* nbx, int* nby)
+{
+    if (!g || !nbx || !nby) return fail(RIFT_E_ARG, "rift_get_tile_dims: null argument");
+    *nbx = g->d.nbx;
+    *nby = g->d.nby;
+    return RIFT_OK;
+}
+
+int rift_get_rates(rift_grid* g, float* out3)
+{
+    if (!g || !out3) return fail(RIFT_E_ARG, "rift_get_rates: null argument");
+    Scalars sc;
+    HIPCHK(hipMemcpyAsync(&sc, g->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    out3[0] = sc.rain_rate;
+    out3[1] = sc.drain_rate;
+    out3[2] = sc.src_cell_rate;
+    return RIFT_OK;
+}
+
+int rift_sum_interior(rift_grid* g, int field, double* out)
+{
+    if (!g || !out) return fail(RIFT_E_ARG, "rift_sum_interior: null argument");
+    float* p = field_plane(g, field);
+    if (!p) return fail(RIFT_E_ARG, "rift_sum_interior: field not available");
+    HIPCHK(hipMemsetAsync(g->dsum, 0, sizeof(double), g->stream));
+    launch_sum_interior(g->d, p, 0, g->dsum, g->stream);
+    int rc = check_launch("sum_interior");
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, g->dsum, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_mass_balance(rift_grid* g, double* out6)
+{
+    if (!g || !out6) return fail(RIFT_E_ARG, "rift_mass_balance: null argument");
+    const double area = g->d.dx * g->d.dy;
+    double v = 0.0, vf = 0.0;
+    int rc = rift_sum_interior(g, RIFT_DEPTH, &v);
+    if (rc) return rc;
+    if (g->d.F && (rc = rift_sum_interior(g, RIFT_F, &vf))) return rc;
+    Scalars sc;
+    HIPCHK(hipMemcpyAsync(&sc, g->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    out6[0] = v * area;
+    out6[1] = vf * area;
+    out6[2] = sc.v_rain * area;
+    out6[3] = sc.v_src * area;
+    out6[4] = sc.v_drain * area;
+    out6[5] = sc.v_out * area;
+    return RIFT_OK;
+}
+
+int rift_get_stream(rift_grid* g, void** stream)
+{
+    if (!g || !stream) return fail(RIFT_E_ARG, "rift_get_stream: null argument");
+    *stream = (void*)g->stream;
+    return RIFT_OK;
+}
+
+int rift_profile_enable(rift_grid* g, int on)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_profile_enable: null handle");
+    HIPCHK(hipStreamSynchronize(g->stream));
+    for (auto& pe : g->prof_events) {
+        g->event_pool.push_back(pe.second.first);
+        g->event_pool.push_back(pe.second.second);
+    }
+    g->prof_events.clear();
+    g->prof = on != 0;
+    return RIFT_OK;
+}
+
+int rift_kernel_stats(rift_grid* g, int kernel, long* launches, double* total_ms)
+{
+    if (!g || !launches || !total_ms || kernel < 0 || kernel >= kProfClasses)
+        return fail(RIFT_E_ARG, "rift_kernel_stats: bad argument");
+    HIPCHK(hipStreamSynchronize(g->stream));
+    long n = 0;
+    double ms = 0.0;
+    for (auto& pe : g->prof_events) {
+        if (pe.first != kernel) continue;
+        float e = 0.f;
+        HIPCHK(hipEventElapsedTime(&e, pe.second.first, pe.second.second));
+        ms += e;
+        ++n;
+    }
+    *launches = n;
+    *total_ms = ms;
+    return RIFT_OK;
+}
+
+int rift_wet_cell_updates(rift_grid* g, long* last_step, long* total)
+{
+    if (!g || !last_step || !total) return fail(RIFT_E_ARG, "rift_wet_cell_updates: null argument");
+    unsigned long long c[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(c, &g->d.sc->wet_cells, sizeof(c), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    *last_step = (long)c[0];
+    *total = (long)c[1];
+    return RIFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// geometry.cpp -- internal-grid construction of Simulator::InitSimulation
+// (simulator.cu:267-436) and the kappa formula (simulator.cu:281).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/rift_host.h"
+
+extern "C" double rift_kappa(double dx, double dy)
+{
+    return (double)std::sqrt(0.01f * std::fmax(1.f, std::fmin((float)dx, (float)dy)));
+}
+
+extern "C" int rift_geometry(int b_nx, int b_ny, const double* dem, const double* h0c,
+                             const double* nc, double n_const, const double* Kc, double* BX,
+                             double* BY, double* h0, double* n, double* K)
+{
+    if (b_nx < 3 || b_ny < 3 || !dem || !BX || !BY || !n) return RIFT_E_ARG;
+    const int nx = b_nx + 3, ny = b_ny + 3;   // grid_config.cpp:84-85
+    std::memset(BX, 0, sizeof(double) * (size_t)(nx + 1) * ny);
+    std::memset(BY, 0, sizeof(double) * (size_t)nx * (ny + 1));
+    std::memset(n, 0, sizeof(double) * (size_t)nx * ny);
+    if (h0) std::memset(h0, 0, sizeof(double) * (size_t)nx * ny);
+    if (K) std::memset(K, 0, sizeof(double) * (size_t)nx * ny);
+    auto bx = [&](int i, int j) -> double& { return BX[(size_t)j * (nx + 1) + i]; };
+    auto by = [&](int i, int j) -> double& { return BY[(size_t)j * nx + i]; };
+    for (int j = 2; j < ny - 2; ++j) {
+        for (int i = 2; i < nx - 2; ++i) {
+            const int jt = j - 2, it = i - 2;
+            const size_t ll = (size_t)jt * b_nx + it, ul = (size_t)(jt + 1) * b_nx + it;
+            const size_t ur = (size_t)(jt + 1) * b_nx + it + 1, lr = (size_t)jt * b_nx + it + 1;
+            bx(i + 1, j) = 0.5f * (dem[ur] + dem[lr]);
+            bx(i, j) = 0.5f * (dem[ll] + dem[ul]);
+            by(i, j + 1) = 0.5f * (dem[ul] + dem[ur]);
+            by(i, j) = 0.5f * (dem[ll] + dem[lr]);
+            // ghost faces repeat the nearest interior faces (simulator.cu:404-414)
+            if (i == 2) {
+                bx(i - 1, j) = bx(i + 1, j);
+                bx(i - 2, j) = bx(i, j);
+            }
+            if (i == nx - 3) bx(i + 2, j) = bx(i, j);
+            if (j == 2) {
+                by(i, j - 1) = by(i, j + 1);
+                by(i, j - 2) = by(i, j);
+            }
+            if (j == ny - 3) by(i, j + 2) = by(i, j);
+            const size_t c = (size_t)j * nx + i;
+            if (h0 && h0c) h0[c] = 0.25f * (h0c[ur] + h0c[lr] + h0c[ll] + h0c[ul]);
+            n[c] = nc ? 0.25f * (nc[ur] + nc[lr] + nc[ll] + nc[ul]) : n_const;
+            if (K && Kc) {
+                double k = 0.25f * (Kc[ur] + Kc[lr] + Kc[ll] + Kc[ul]);
+                K[c] = k / (3600.f * 1000.f);
+            }
+        }
+    }
+    return RIFT_OK;
+}

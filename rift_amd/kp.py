@@ -1,0 +1,316 @@
+"""ctypes binding of librift_kp.so (include/rift_kp.h, include/rift_host.h).
+
+`Grid` mirrors the reference's Grid host API (grid.h:6-69) over one handle:
+construction = SetDeviceConstants + AllocateGrid, `upload_static` + `init` =
+the uploads of InitSimulation + InitGrid + ApplyBoundaries, `step` =
+RunSimulation's body, and the phase methods map one-to-one onto
+Grow / ComputeFluxes / ComputeTimestep / Integrate_1 / Integrate_2 /
+ApplyBoundaries.  There is no CPU fallback: without the built library every
+call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librift_kp.so")
+
+HYDROGRAPH = 1 << 0
+RAIN_AVG = 1 << 1
+RAIN_GRID = 1 << 2
+DRAIN_AVG = 1 << 3
+DRAIN_GRID = 1 << 4
+INFILTRATION = 1 << 5
+SURGE_GRID = 1 << 6
+EULER = 1 << 7
+SAVE_MAX = 1 << 8
+SAVE_ARRIVAL = 1 << 9
+TILE_ACCEL = 1 << 10
+H_INIT = 1 << 11
+RAIN_INTERP = 1 << 12
+TEXTBOOK = 1 << 16
+
+FIELDS = ["w", "hu", "hv", "depth", "bc", "n", "K", "F", "dw", "dhu", "dhv", "G", "dF",
+          "h_max", "q_max", "t_wet", "t_peak", "t_dry", "surge", "rainmap", "drainmap", "q"]
+SER_HYETO, SER_DRAIN, SER_HYDRO = 0, 1, 2
+MAP_RAIN, MAP_DRAIN, MAP_SURGE = 0, 1, 2
+KERNEL_CLASSES = ["flux1", "stage2_flux", "integrate", "boundaries", "grow", "timestep"]
+
+# (name, restype, argtypes) of every exported entry point
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_I = C.POINTER(C.c_int)
+_U8 = C.POINTER(C.c_ubyte)
+_VP = C.c_void_p
+
+
+class RiftDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("dx", C.c_double), ("dy", C.c_double),
+                ("kappa", C.c_double), ("courant", C.c_double), ("psi", C.c_double),
+                ("dtheta", C.c_double), ("nodata", C.c_double), ("flags", C.c_uint),
+                ("device", C.c_int)]
+
+
+SIGNATURES = [
+    ("rift_last_error", C.c_char_p, []),
+    ("rift_create", C.c_int, [C.POINTER(RiftDesc), C.POINTER(_VP)]),
+    ("rift_destroy", C.c_int, [_VP]),
+    ("rift_upload_static", C.c_int, [_VP, _D, _D, _D, _D, _D]),
+    ("rift_init", C.c_int, [_VP, C.c_double]),
+    ("rift_set_series", C.c_int, [_VP, C.c_int, _F, _F, C.c_int]),
+    ("rift_set_source_cells", C.c_int, [_VP, _I, C.c_long]),
+    ("rift_set_source_value", C.c_int, [_VP, C.c_double]),
+    ("rift_set_map", C.c_int, [_VP, C.c_int, C.c_int, _D]),
+    ("rift_interp_map", C.c_int, [_VP, C.c_int, C.c_double]),
+    ("rift_copy_map", C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
+    ("rift_step", C.c_int, [_VP, C.c_int]),
+    ("rift_update_source", C.c_int, [_VP]),
+    ("rift_grow", C.c_int, [_VP]),
+    ("rift_compute_fluxes", C.c_int, [_VP]),
+    ("rift_compute_timestep", C.c_int, [_VP]),
+    ("rift_integrate_1", C.c_int, [_VP]),
+    ("rift_integrate_2", C.c_int, [_VP]),
+    ("rift_apply_boundaries", C.c_int, [_VP]),
+    ("rift_sync", C.c_int, [_VP]),
+    ("rift_get_time", C.c_int, [_VP, _D, _D]),
+    ("rift_set_time", C.c_int, [_VP, C.c_double, C.c_double]),
+    ("rift_download", C.c_int, [_VP, C.c_int, _D]),
+    ("rift_upload", C.c_int, [_VP, C.c_int, _D]),
+    ("rift_get_tiles", C.c_int, [_VP, _U8, _U8, _I]),
+    ("rift_set_tiles", C.c_int, [_VP, _U8, _U8]),
+    ("rift_get_tile_speeds", C.c_int, [_VP, _F, _F]),
+    ("rift_get_tile_dims", C.c_int, [_VP, _I, _I]),
+    ("rift_get_rates", C.c_int, [_VP, _F]),
+    ("rift_sum_interior", C.c_int, [_VP, C.c_int, _D]),
+    ("rift_mass_balance", C.c_int, [_VP, _D]),
+    ("rift_get_stream", C.c_int, [_VP, C.POINTER(_VP)]),
+    ("rift_profile_enable", C.c_int, [_VP, C.c_int]),
+    ("rift_kernel_stats", C.c_int, [_VP, C.c_int, C.POINTER(C.c_long), _D]),
+    ("rift_wet_cell_updates", C.c_int, [_VP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+    ("rift_kappa", C.c_double, [C.c_double, C.c_double]),
+    ("rift_geometry", C.c_int, [C.c_int, C.c_int, _D, _D, _D, C.c_double, _D, _D, _D, _D, _D, _D]),
+]
+
+_lib = None
+
+
+class RiftError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the in-tree library; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RiftError(f"{LIB_PATH} is missing: run `make -C rift_amd` (or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RiftError(f"{what} failed ({rc}): {load().rift_last_error().decode()}")
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def kappa(dx, dy):
+    return load().rift_kappa(dx, dy)
+
+
+def geometry(dem, h0=None, n=None, n_const=0.0, K=None):
+    """InitSimulation's internal fields from corner rasters (row 0 = south).
+    Returns dict with BX (ny, nx+1), BY (ny+1, nx), h0, n, K (ny, nx)."""
+    dem = np.ascontiguousarray(dem, dtype=np.float64)
+    b_ny, b_nx = dem.shape
+    nx, ny = b_nx + 3, b_ny + 3
+    arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (h0, n, K)]
+    BX = np.empty((ny, nx + 1))
+    BY = np.empty((ny + 1, nx))
+    H0 = np.empty((ny, nx)) if h0 is not None else None
+    N = np.empty((ny, nx))
+    KK = np.empty((ny, nx)) if K is not None else None
+    _check(load().rift_geometry(b_nx, b_ny, _dp(dem), _dp(arrs[0]), _dp(arrs[1]), float(n_const),
+                                _dp(arrs[2]), _dp(BX), _dp(BY), _dp(H0), _dp(N), _dp(KK)),
+           "rift_geometry")
+    return {"BX": BX, "BY": BY, "h0": H0, "n": N, "K": KK, "nx": nx, "ny": ny}
+
+
+class Grid:
+    """One simulation domain on one GPU (a rift_grid handle)."""
+
+    def __init__(self, nx, ny, dx, *, dy=None, kappa_=None, courant=0.25, psi=0.0, dtheta=0.0,
+                 nodata=-9999.0, flags=TILE_ACCEL, device=0):
+        L = load()
+        dy = dx if dy is None else dy
+        k = L.rift_kappa(dx, dy) if kappa_ is None else kappa_
+        self.desc = RiftDesc(nx, ny, dx, dy, k, courant, psi, dtheta, nodata, flags, device)
+        h = _VP()
+        _check(L.rift_create(C.byref(self.desc), C.byref(h)), "rift_create")
+        self.h = h
+        self.nx, self.ny, self.flags = nx, ny, flags
+        a, b = C.c_int(), C.c_int()
+        _check(L.rift_get_tile_dims(self.h, C.byref(a), C.byref(b)), "rift_get_tile_dims")
+        self.nbx, self.nby = a.value, b.value
+        self.ntiles = self.nbx * self.nby
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().rift_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- setup ---------------------------------------------------------------
+    def upload_static(self, BX, BY, n, K=None, h0=None):
+        keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (BX, BY, n, K, h0)]
+        _check(load().rift_upload_static(self.h, *[_dp(a) for a in keep]), "rift_upload_static")
+
+    def init(self, t0=0.0):
+        _check(load().rift_init(self.h, t0), "rift_init")
+
+    def set_series(self, which, time_s, values):
+        t = np.ascontiguousarray(time_s, dtype=np.float32)
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        _check(load().rift_set_series(self.h, which, t.ctypes.data_as(_F), v.ctypes.data_as(_F),
+                                      len(t)), "rift_set_series")
+
+    def set_source_cells(self, idx):
+        a = np.ascontiguousarray(idx, dtype=np.int32)
+        _check(load().rift_set_source_cells(self.h, a.ctypes.data_as(_I), len(a)),
+               "rift_set_source_cells")
+
+    def set_source_value(self, q):
+        _check(load().rift_set_source_value(self.h, q), "rift_set_source_value")
+
+    def set_map(self, which, plane, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(load().rift_set_map(self.h, which, plane, _dp(a)), "rift_set_map")
+
+    def interp_map(self, which, factor):
+        _check(load().rift_interp_map(self.h, which, factor), "rift_interp_map")
+
+    def copy_map(self, which, dst, src):
+        _check(load().rift_copy_map(self.h, which, dst, src), "rift_copy_map")
+
+    # --- stepping ------------------------------------------------------------
+    def step(self, n=1):
+        _check(load().rift_step(self.h, n), "rift_step")
+
+    def phase(self, name):
+        _check(getattr(load(), "rift_" + name)(self.h), "rift_" + name)
+
+    def sync(self):
+        _check(load().rift_sync(self.h), "rift_sync")
+
+    # --- readback ------------------------------------------------------------
+    @property
+    def time(self):
+        t, dt = C.c_double(), C.c_double()
+        _check(load().rift_get_time(self.h, C.byref(t), C.byref(dt)), "rift_get_time")
+        return t.value, dt.value
+
+    def set_time(self, t, dt):
+        _check(load().rift_set_time(self.h, t, dt), "rift_set_time")
+
+    def field(self, name):
+        out = np.empty((self.ny, self.nx))
+        _check(load().rift_download(self.h, FIELDS.index(name), _dp(out)), "rift_download " + name)
+        return out
+
+    def set_field(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        _check(load().rift_upload(self.h, FIELDS.index(name), _dp(a)), "rift_upload " + name)
+
+    def tiles(self):
+        wet = np.empty(self.ntiles, dtype=np.uint8)
+        act = np.empty(self.ntiles, dtype=np.uint8)
+        n = C.c_int()
+        _check(load().rift_get_tiles(self.h, wet.ctypes.data_as(_U8), act.ctypes.data_as(_U8),
+                                     C.byref(n)), "rift_get_tiles")
+        return wet, act, n.value
+
+    def set_tiles(self, wet, act):
+        w = np.ascontiguousarray(wet, dtype=np.uint8)
+        a = np.ascontiguousarray(act, dtype=np.uint8)
+        _check(load().rift_set_tiles(self.h, w.ctypes.data_as(_U8), a.ctypes.data_as(_U8)),
+               "rift_set_tiles")
+
+    def active_set(self):
+        wet, act, n = self.tiles()
+        return set(int(t) for t in np.nonzero(act)[0])
+
+    def tile_speeds(self):
+        mx = np.empty(self.ntiles, dtype=np.float32)
+        my = np.empty(self.ntiles, dtype=np.float32)
+        _check(load().rift_get_tile_speeds(self.h, mx.ctypes.data_as(_F), my.ctypes.data_as(_F)),
+               "rift_get_tile_speeds")
+        return mx, my
+
+    def rates(self):
+        out = np.empty(3, dtype=np.float32)
+        _check(load().rift_get_rates(self.h, out.ctypes.data_as(_F)), "rift_get_rates")
+        return out
+
+    def sum_interior(self, name):
+        out = C.c_double()
+        _check(load().rift_sum_interior(self.h, FIELDS.index(name), C.byref(out)), "rift_sum_interior")
+        return out.value
+
+    def mass_balance(self):
+        out = np.empty(6)
+        _check(load().rift_mass_balance(self.h, _dp(out)), "rift_mass_balance")
+        return out
+
+    def stream(self):
+        s = _VP()
+        _check(load().rift_get_stream(self.h, C.byref(s)), "rift_get_stream")
+        return s.value
+
+    def profile(self, on=True):
+        _check(load().rift_profile_enable(self.h, 1 if on else 0), "rift_profile_enable")
+
+    def kernel_stats(self, cls):
+        n, ms = C.c_long(), C.c_double()
+        _check(load().rift_kernel_stats(self.h, KERNEL_CLASSES.index(cls), C.byref(n), C.byref(ms)),
+               "rift_kernel_stats")
+        return n.value, ms.value
+
+    def wet_cells(self):
+        """(wet-cell updates of the last step, cumulative since init)."""
+        a, b = C.c_long(), C.c_long()
+        _check(load().rift_wet_cell_updates(self.h, C.byref(a), C.byref(b)), "rift_wet_cell_updates")
+        return a.value, b.value
+
+    def interior(self, a):
+        return a[2:self.ny - 2, 2:self.nx - 2]
+
+
+def grid_from_rasters(dem, dx, *, h0=None, n=None, n_const=0.0, K=None, psi=0.0, dtheta=0.0,
+                      courant=0.25, flags=TILE_ACCEL, t0=0.0, device=0):
+    """Build, upload and initialise a Grid from corner rasters (row 0 = south)."""
+    if h0 is not None:
+        flags |= H_INIT
+    if K is not None:
+        flags |= INFILTRATION
+    geo = geometry(dem, h0=h0, n=n, n_const=n_const, K=K)
+    g = Grid(geo["nx"], geo["ny"], dx, courant=courant, psi=psi, dtheta=dtheta, flags=flags,
+             device=device)
+    g.upload_static(geo["BX"], geo["BY"], geo["n"], geo["K"], geo["h0"])
+    g.init(t0)
+    return g

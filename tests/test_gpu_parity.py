@@ -1,0 +1,150 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances (FP32 device state vs the FP64 oracle of the reference scheme):
+  - face fluxes / dU of one flux pass: |diff| <= 2e-5 * max|dU| + 1e-7
+  - dt of every step: relative 2e-5
+  - depth after a whole run: max |dh| <= 1e-4 m (the target bound)
+  - wet / active tile sets: identical at every step (integer work)
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from rift_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(rift, dem, dx, *, h0=None, n=None, n_const=0.0, K=None, psi=0.0, dtheta=0.0,
+          courant=0.25, oflags=O.TILE_ACCEL, mode=O.FAITHFUL, t0=0.0):
+    gflags = rift.TILE_ACCEL if (oflags & O.TILE_ACCEL) else 0
+    for ob, gb in ((O.RAIN_AVG, rift.RAIN_AVG), (O.EULER, rift.EULER), (O.SAVE_MAX, rift.SAVE_MAX),
+                   (O.SAVE_ARRIVAL, rift.SAVE_ARRIVAL), (O.HYDROGRAPH, rift.HYDROGRAPH),
+                   (O.DRAIN_AVG, rift.DRAIN_AVG)):
+        if oflags & ob:
+            gflags |= gb
+    if mode == O.TEXTBOOK:
+        gflags |= rift.TEXTBOOK
+    o = O.Oracle(dem, dx, h0=h0, n=n, n_const=n_const, K=K, psi=psi, dtheta=dtheta,
+                 courant=courant, flags=oflags, mode=mode, t0=t0)
+    g = rift.grid_from_rasters(dem, dx, h0=h0, n=n, n_const=n_const, K=K, psi=psi,
+                               dtheta=dtheta, courant=courant, flags=gflags, t0=t0)
+    return o, g
+
+
+def _active_oracle(o):
+    wet, act, n = o.tiles()
+    return {int(v) for v in act if v > -1}
+
+
+def test_first_flux_dam_break(rift):
+    cfg = synth.dam_break(256)
+    o, g = _pair(rift, cfg["dem"], 1.0, h0=cfg["h0"])
+    np.testing.assert_allclose(g.interior(g.field("depth")), o.interior(o.depth()), atol=1e-7)
+    for ph in ("grow", "compute_fluxes"):
+        o.phase(ph)
+        g.phase(ph)
+    assert g.active_set() == _active_oracle(o)
+    for f in ("dw", "dhu", "dhv"):
+        a, b = g.interior(g.field(f)), o.interior(o.field(f))
+        tol = 2e-5 * max(np.abs(b).max(), 1e-3) + 1e-7
+        assert np.abs(a - b).max() <= tol, (f, np.abs(a - b).max(), tol)
+    gmx, gmy = g.tile_speeds()
+    omx, omy = o.tile_speeds()
+    np.testing.assert_allclose(gmx, omx, rtol=2e-6, atol=1e-7)
+    np.testing.assert_allclose(gmy, omy, rtol=2e-6, atol=1e-7)
+    g.close()
+
+
+def _run_compare(o, g, nsteps=None, t_end=None, check_tiles=True, dt_rtol=2e-5):
+    k = 0
+    max_dt_err = 0.0
+    while True:
+        if nsteps is not None and k >= nsteps:
+            break
+        if t_end is not None and o.time[0] >= t_end:
+            break
+        o.step()
+        g.step()
+        to, dto = o.time
+        tg, dtg = g.time
+        max_dt_err = max(max_dt_err, abs(dtg - dto) / dto)
+        assert abs(dtg - dto) <= dt_rtol * dto, (k, dto, dtg)
+        if check_tiles:
+            assert g.active_set() == _active_oracle(o), k
+            gw = g.tiles()[0]
+            ow = o.tiles()[0]
+            np.testing.assert_array_equal(gw, ow)
+        k += 1
+    return k, max_dt_err
+
+
+def test_dam_break_full_run(rift):
+    """256 x 256 dam break to t = 20 s (about 340 steps)."""
+    cfg = synth.dam_break(256)
+    o, g = _pair(rift, cfg["dem"], 1.0, h0=cfg["h0"])
+    steps, dterr = _run_compare(o, g, t_end=20.0)
+    assert steps > 300
+    dh = np.abs(g.interior(g.field("depth")) - o.interior(o.depth())).max()
+    assert dh <= 1e-4, dh
+    for f in ("hu", "hv"):
+        d = np.abs(g.interior(g.field(f)) - o.interior(o.field(f))).max()
+        assert d <= 2e-4, (f, d)
+    g.close()
+
+
+def test_dam_break_textbook_mode(rift):
+    cfg = synth.dam_break(128)
+    o, g = _pair(rift, cfg["dem"], 1.0, h0=cfg["h0"], mode=O.TEXTBOOK)
+    _run_compare(o, g, nsteps=120)
+    dh = np.abs(g.interior(g.field("depth")) - o.interior(o.depth())).max()
+    assert dh <= 1e-4, dh
+    g.close()
+
+
+def test_rain_basin_infiltration_friction(rift):
+    """The rain_basin recipe at 128 x 128: rain, Green-Ampt, Manning, dry start."""
+    cfg = synth.rain_basin(128, seed=2)
+    o, g = _pair(rift, cfg["dem"], cfg["dx"], n_const=cfg["n_const"], K=cfg["K"],
+                 psi=cfg["psi"], dtheta=cfg["dtheta"], oflags=O.TILE_ACCEL | O.RAIN_AVG)
+    t, v = synth.series_hours_to_seconds(cfg["hyeto_hours"], cfg["hyeto_mmhr"])
+    o.set_series_hours(O.SER_HYETO, cfg["hyeto_hours"], cfg["hyeto_mmhr"])
+    g.set_series(rift.SER_HYETO, t, v)
+    _run_compare(o, g, nsteps=150)
+    np.testing.assert_array_equal(g.rates()[0], o.rates()[0])
+    dh = np.abs(g.interior(g.field("depth")) - o.interior(o.depth())).max()
+    assert dh <= 1e-4, dh
+    dF = np.abs(g.interior(g.field("F")) - o.interior(o.field("F"))).max()
+    assert dF <= 1e-6, dF
+    g.close()
+
+
+def test_euler_mode_and_diagnostics(rift):
+    cfg = synth.dam_break(64)
+    flags = O.TILE_ACCEL | O.EULER | O.SAVE_MAX | O.SAVE_ARRIVAL
+    o, g = _pair(rift, cfg["dem"], 1.0, h0=cfg["h0"], n_const=0.02, oflags=flags)
+    _run_compare(o, g, nsteps=60)
+    for f, tol in (("depth", 1e-4), ("h_max", 1e-4), ("q_max", 2e-4)):
+        a = g.interior(g.field(f))
+        b = o.interior(o.depth() if f == "depth" else o.field(f))
+        assert np.abs(a - b).max() <= tol, f
+    g.close()
+
+
+def test_hydrograph_source_rectangle(rift):
+    N = 64
+    dem = np.zeros((N + 1, N + 1))
+    nx = N + 3
+    idx = [j * nx + i for i in range(30, 34) for j in range(20, 23)]
+    hours = np.array([0.0, 0.01, 0.05], dtype=np.float32)
+    cfs = np.array([0.0, 50.0, 50.0], dtype=np.float32)
+    o, g = _pair(rift, dem, 2.0, n_const=0.03, oflags=O.TILE_ACCEL | O.HYDROGRAPH)
+    o.set_sources(idx)
+    o.set_series_hours(O.SER_HYDRO, hours, cfs)
+    g.set_source_cells(idx)
+    t, v = synth.series_hours_to_seconds(hours, cfs)
+    g.set_series(rift.SER_HYDRO, t, v)
+    _run_compare(o, g, nsteps=80)
+    dh = np.abs(g.interior(g.field("depth")) - o.interior(o.depth())).max()
+    assert dh <= 1e-4, dh
+    g.close()
