@@ -157,6 +157,12 @@ int rift_integrate_1(rift_grid* g);      /* Integrate_1, grid.cu:1026       */
 int rift_integrate_2(rift_grid* g);      /* Integrate_2, grid.cu:1173       */
 int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
 
+/* Options (rift_set_option).  RIFT_OPT_STEP_PATH: 0 = fused marching step
+ * (default), 1 = one kernel per reference phase (Grow, ComputeFluxes,
+ * Integrate_1, ApplyBoundaries, ...); both give the same results. */
+#define RIFT_OPT_STEP_PATH 1
+int rift_set_option(rift_grid* g, int option, int value);
+
 /* Wait for all queued work of the handle. */
 int rift_sync(rift_grid* g);
 

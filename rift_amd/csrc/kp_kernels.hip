@@ -45,6 +45,7 @@ __global__ void k_reset_counters(Dev d)
 {
     if (threadIdx.x != 0) return;
     d.sc->n_active = 0;
+    d.sc->n_units = 0;
     d.sc->wet_cells = 0ull;
 }
 
@@ -313,6 +314,11 @@ __global__ __launch_bounds__(256) void k_timestep(Dev d)
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.ntiles; t += gridDim.x * blockDim.x) {
         vx = fmaxf(vx, d.mx[t]);
         vy = fmaxf(vy, d.my[t]);
+        // active tiles accumulate the next flux pass's maxima from zero
+        if (d.act[t]) {
+            d.mx[t] = 0.f;
+            d.my[t] = 0.f;
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         vx = fmaxf(vx, __shfl_xor(vx, off));
@@ -341,6 +347,7 @@ __global__ __launch_bounds__(256) void k_timestep(Dev d)
         double c_y = d.dy / (double)fmaxf((float)(max_y / d.courant), (float)d.kappa_d);
         double dt = (c_x < c_y) ? c_x : c_y;
         if (sc->t == 0.0) dt = (double)0.000001f;
+        sc->dt_prev = sc->dt;
         sc->dt = dt;
         sc->t += dt;
         // FP64 mass accounting of the scalar inflows applied this step
@@ -582,6 +589,11 @@ void launch_track(const Dev& d, hipStream_t s)
 {
     int grid = d.ntiles < 8192 ? d.ntiles : 8192;
     hipLaunchKernelGGL(k_track, dim3(grid), dim3(256), 0, s, d);
+}
+
+void launch_reset_counters(const Dev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, d);
 }
 
 void launch_grow(const Dev& d, hipStream_t s)

@@ -35,6 +35,8 @@ struct Scalars {
     double hydro_value;          // external discharge, m^3/s
     int n_active;
     unsigned mx_bits, my_bits, ticket;
+    int n_units;                     // active marching units of this step
+    double dt_prev;                  // dt of the stage-1 sources (previous step's dt)
     unsigned long long wet_cells;    // interior cells of active tiles (this step)
     unsigned long long wcu_total;    // cumulative wet-cell updates since init
     // FP64 mass accounting (m^3 / (dx dy) until scaled on the host)
@@ -49,6 +51,16 @@ struct Dev {
     float *hmax, *qmax, *twet, *tpeak, *tdry;
     unsigned char *wet, *act;
     int* list;
+    // fused step: per-tile buffer select (state of the tile's cells lives in
+    // h/hu/hv when 0, in h0/hu0/hv0 when 1), pending flips, the stage-2
+    // contributor word (count << 8 | wet contributors), the active units
+    unsigned char *sel, *flip;
+    unsigned* pending;
+    int* ulist;
+    int nstrips, nurows, nunits;
+    long long soff;              // h0 - h (elements): offset of the second buffer
+    int n_uniform;
+    float n_const;
     float *mx, *my;
     Scalars* sc;
     SeriesTable* ser;            // [3]: hyetograph, drain, hydrograph
@@ -69,6 +81,7 @@ __host__ __device__ inline size_t cell(const Dev& d, int i, int j)
 void launch_begin_step(const Dev& d, hipStream_t s);
 void launch_track(const Dev& d, hipStream_t s);
 void launch_grow(const Dev& d, hipStream_t s);
+void launch_reset_counters(const Dev& d, hipStream_t s);
 void launch_flux(const Dev& d, int stage, hipStream_t s, int grid);
 void launch_timestep(const Dev& d, hipStream_t s);
 void launch_integrate1(const Dev& d, hipStream_t s, int grid);
@@ -76,5 +89,12 @@ void launch_integrate2(const Dev& d, hipStream_t s, int grid);
 void launch_boundaries(const Dev& d, hipStream_t s);
 void launch_interp_map(const Dev& d, int which, float factor, hipStream_t s);
 void launch_sum_interior(const Dev& d, const float* x, int mode, double* out, hipStream_t s);
+
+// Fused step (kp_step2.hip).  One wavefront marches one unit: 60 output
+// columns (lanes 2..61) by one 14-row tile row.
+constexpr int kStrip = 60;
+void launch_grow2(const Dev& d, hipStream_t s);
+void launch_march(const Dev& d, int stage, hipStream_t s);
+void launch_canonicalize(const Dev& d, hipStream_t s);
 
 }  // namespace rift

@@ -1,0 +1,496 @@
+// kp_step2.hip -- the fused Heun step: two marching kernels per step.
+//
+//   grow2      tile dilation (+ pending buffer flips, per-tile resets) and
+//              compaction of the active marching units
+//   march<1>   stage-1 flux + sources of U^n            -> dU, tile speeds
+//   timestep   dt from all tile speeds                   (kp_kernels.hip)
+//   march<2>   U* = Integrate_1(U^n, dU) on the fly, flux + sources of U*,
+//              U^{n+1} = (U^n + U* + dt L(U*))/2, infiltration, diagnostics,
+//              tile wet flags                            -> U^{n+1} in the
+//              other buffer of the tile
+//
+// A unit is one wavefront: lane L holds column c = 60*s + L of strip s and
+// marches the 18 rows j0-2 .. j0+15 of tile row r, keeping three rows in
+// registers; x neighbours come from DPP wavefront shifts, so no LDS is used.
+// Lanes 2..61 produce output columns 60s+2 .. 60s+61.
+//
+// Ghost cells are never stored: a lane or row outside the interior loads the
+// interior cell the reference's ApplyBoundaries copies from (grid.cu:1325-
+// 1406) and applies the same critical-depth transform, which is what the
+// ghost holds whenever the reference computes a flux.
+//
+// Stage 2 cannot update U in place (neighbouring units read it as halo), so
+// every 16 x 14 tile owns a select bit: its current state lives in h/hu/hv or
+// in h0/hu0/hv0, and stage 2 writes the other one; the bit flips at the next
+// grow2.  Inactive tiles are never touched, so no copies are needed.
+#include <hip/hip_runtime.h>
+
+#include "kp_device.hpp"
+#include "kp_kernels.hpp"
+
+#include "../../include/rift_kp.h"
+
+namespace rift {
+
+// lane L receives lane L-1's value (wave_shr:1) / lane L+1's (wave_shl:1)
+__device__ __forceinline__ float from_left(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ bool dilated(const Dev& d, int t)
+{
+    const int bx = t % d.nbx, by = t / d.nbx, nbx = d.nbx;
+    const unsigned char* w = d.wet;
+    return w[t] || (bx > 0 && w[t - 1]) || (bx < nbx - 1 && w[t + 1]) ||
+           (by > 0 && w[t - nbx]) || (by < d.nby - 1 && w[t + nbx]) || t == 0;
+}
+
+// tile columns covered by the output columns of strip s
+__device__ __forceinline__ void strip_tiles(const Dev& d, int s, int& txa, int& txb)
+{
+    const int c0 = kStrip * s + 2;
+    const int c1 = min(c0 + kStrip - 1, d.nx - 3);
+    txa = (c0 - 2) / kTileW;
+    txb = (c1 - 2) / kTileW;
+}
+
+// ---------------------------------------------------------------------------
+// grow2: threads [0, ntiles) handle tiles, [ntiles, ntiles+nunits) units.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grow2(Dev d)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool is_tile = t < d.ntiles;
+    const bool is_unit = t >= d.ntiles && t < d.ntiles + d.nunits;
+    bool a = false, ua = false;
+    unsigned long long cells = 0ull;
+    if (is_tile) {
+        if (d.flip[t]) {   // stage 2 of the last step wrote the other buffer
+            d.sel[t] ^= 1;
+            d.flip[t] = 0;
+        }
+        a = dilated(d, t);
+        d.act[t] = a ? 1 : 0;
+        if (a) {
+            const int bx = t % d.nbx, by = t / d.nbx;
+            const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
+            const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
+            unsigned units = 0;
+            if (i1 > i0 && j1 > j0) {
+                cells = (unsigned long long)((i1 - i0) * (j1 - j0));
+                units = (unsigned)((i1 - 1 - 2) / kStrip - (i0 - 2) / kStrip + 1);
+            }
+            d.mx[t] = 0.f;
+            d.my[t] = 0.f;
+            // contributor count in bits 8.., wet contributors in bits 0..7
+            d.pending[t] = units << 8;
+        }
+    }
+    if (is_unit) {
+        const int u = t - d.ntiles;
+        const int s = u % d.nstrips, r = u / d.nstrips;
+        int txa, txb;
+        strip_tiles(d, s, txa, txb);
+        for (int tx = txa; tx <= txb; ++tx) ua |= dilated(d, r * d.nbx + tx);
+    }
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long m = __ballot(a), mu = __ballot(ua);
+    for (int off = 32; off > 0; off >>= 1) cells += __shfl_xor(cells, off);
+    int base = 0, ubase = 0;
+    if (lane == 0) {
+        int cnt = __popcll(m), ucnt = __popcll(mu);
+        if (cnt) {
+            base = atomicAdd(&d.sc->n_active, cnt);
+            atomicAdd(&d.sc->wet_cells, cells);
+            atomicAdd(&d.sc->wcu_total, cells);
+        }
+        if (ucnt) ubase = atomicAdd(&d.sc->n_units, ucnt);
+    }
+    base = __shfl(base, 0);
+    ubase = __shfl(ubase, 0);
+    if (a) d.list[base + __popcll(m & lt)] = t;
+    if (ua) d.ulist[ubase + __popcll(mu & lt)] = t - d.ntiles;
+}
+
+// ---------------------------------------------------------------------------
+// marching kernels
+// ---------------------------------------------------------------------------
+struct Raw {                 // global loads of one row, issued a row ahead
+    float h, hu, hv, b, by;
+    float dh, dhu, dhv, sg, n;   // stage 2
+};
+
+struct Row {
+    float h, hu, hv, b;     // state used by the flux (U^n in stage 1, U* in 2)
+    float hraw;             // stage 2: U^n depth as stored (stage-1 sources)
+    float hn, hun, hvn;     // stage 2: clamped U^n of the cell (own-cell Heun)
+};
+
+struct Src {                 // own-cell inputs of an output row
+    float bx, K, F, rain, drain;
+};
+
+// Per-lane constants of one unit.  The rows of a unit touch tile rows r-1,
+// r, r+1 only; their buffer selects and activity flags are read once.
+struct Lane {
+    int c, sc, txo, r;
+    bool gx;
+    unsigned char sel[3], act[3];
+};
+
+__device__ __forceinline__ int tile_row_slot(const Lane& L, int sr)
+{
+    const int tr = (sr - 2) / kTileH;
+    return tr - L.r + 1;     // 0, 1 or 2
+}
+
+template <int Stage>
+__device__ __forceinline__ Raw fetch(const Dev& d, const Lane& L, int j)
+{
+    const int sr = min(max(j, 2), d.ny - 3);
+    const int slot = tile_row_slot(L, sr);
+    const size_t q = cell(d, L.sc, sr);
+    const size_t qs = q + (L.sel[slot] ? (size_t)d.soff : 0);
+    Raw r;
+    r.h = d.h[qs];
+    r.hu = d.hu[qs];
+    r.hv = d.hv[qs];
+    r.b = d.BC[q];
+    r.by = d.BY[cell(d, L.c, j)];
+    if constexpr (Stage == 2) {
+        r.dh = d.dh[q];
+        r.dhu = d.dhu[q];
+        r.dhv = d.dhv[q];
+        r.sg = (d.flags & RIFT_SURGE_GRID) ? d.map[RIFT_MAP_SURGE][0][q] : 0.f;
+        r.n = d.n_uniform ? d.n_const : d.n[q];
+    }
+    return r;
+}
+
+__device__ __forceinline__ Src fetch_src(const Dev& d, const Lane& L, int j)
+{
+    const size_t q = cell(d, L.c, j);
+    Src s;
+    s.bx = d.BX[q];
+    s.K = (d.flags & RIFT_INFILTRATION) ? d.K[q] : 0.f;
+    s.F = (d.flags & RIFT_INFILTRATION) ? d.F[q] : 0.f;
+    s.rain = (d.flags & RIFT_RAIN_GRID) ? d.map[RIFT_MAP_RAIN][0][q] : 0.f;
+    s.drain = (d.flags & RIFT_DRAIN_GRID) ? d.map[RIFT_MAP_DRAIN][0][q] : 0.f;
+    return s;
+}
+
+template <int Stage>
+__device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& raw, int j, float dt,
+                                      float inv_dt)
+{
+    const int sr = min(max(j, 2), d.ny - 3);
+    const bool gy = (j < 2) || (j > d.ny - 3);
+    float h = raw.h, hu = raw.hu, hv = raw.hv;
+    Row r;
+    r.b = raw.b;
+    if constexpr (Stage == 2) {
+        r.hraw = h;
+        r.hn = (h > kEps) ? h : 0.f;
+        r.hun = hu;
+        r.hvn = hv;
+        if (L.act[tile_row_slot(L, sr)]) {
+            // Integrate_1_k (grid.cu:856-922) of this cell from U^n and dU;
+            // G is recomputed from U^n exactly as the stage-1 flux forms it
+            const float G = manning_g(raw.n, hu, hv, h, d.kappa);
+            const float hcl = r.hn;
+            float dh = raw.dh;
+            if ((d.flags & RIFT_SURGE_GRID) && raw.sg != d.nodata && raw.sg > 0.f)
+                dh = (raw.sg - (hcl + dh * dt)) * inv_dt;
+            const float hnew = hcl + dh * dt;
+            const float den = 1.f - dt * G;
+            hu = (hu + raw.dhu * dt) / den;
+            hv = (hv + raw.dhv * dt) / den;
+            h = (hnew > kEps) ? hnew : 0.f;
+        }
+    }
+    (void)inv_dt;
+    if (L.gx) {      // ApplyBoundariesX_k: epsilon-clamped critical depth
+        const float H = fminf(critical_depth(hu), h);
+        h = (H > kEps) ? H : 0.f;
+    } else if (gy) { // ApplyBoundariesY_k: no clamp
+        h = fminf(critical_depth(hv), h);
+    }
+    r.h = h;
+    r.hu = hu;
+    r.hv = hv;
+    return r;
+}
+
+template <int Stage, bool Textbook>
+__global__ __launch_bounds__(256) void k_march(Dev d)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int nwaves = gridDim.x * 4;
+    const int nunits = d.sc->n_units;
+    const float dt = (float)d.sc->dt, inv_dt = 1.f / dt;
+    const float dtp = (float)d.sc->dt_prev, inv_dtp = 1.f / dtp;
+    const float kappa = d.kappa;
+    const unsigned fl = d.flags;
+    const float rain_rate = d.sc->rain_rate, drain_rate = d.sc->drain_rate;
+    const float src_rate = d.sc->src_cell_rate;
+    const float tnow = (float)d.sc->t;
+    const float inv_dx = d.inv_dx, inv_dy = d.inv_dy;
+
+    for (int k = wave; k < nunits; k += nwaves) {
+        const int u = d.ulist[k];
+        const int s = u % d.nstrips, rr = u / d.nstrips;
+        Lane L;
+        L.c = kStrip * s + lane;
+        L.sc = min(max(L.c, 2), d.nx - 3);
+        L.gx = (L.c < 2) || (L.c > d.nx - 3);
+        L.txo = (L.sc - 2) / kTileW;
+        L.r = rr;
+        for (int q = 0; q < 3; ++q) {
+            const int tr = min(max(rr - 1 + q, 0), d.nby - 1);
+            const int T = tr * d.nbx + L.txo;
+            L.sel[q] = d.sel[T];
+            L.act[q] = d.act[T];
+        }
+        const bool out_lane = lane >= 2 && lane <= kStrip + 1 && L.c <= d.nx - 3;
+        const bool act_own = L.act[1] != 0;
+        const int j0 = rr * kTileH + 2;
+        const int jend = min(j0 + kTileH, d.ny - 2);
+
+        const Row A = finish<Stage>(d, L, fetch<Stage>(d, L, j0 - 2), j0 - 2, dt, inv_dt);
+        const Raw rB = fetch<Stage>(d, L, j0 - 1);
+        const Raw rC = fetch<Stage>(d, L, j0);
+        const Raw rD = fetch<Stage>(d, L, j0 + 1);
+        Raw rE = fetch<Stage>(d, L, j0 + 2);
+        Src src = fetch_src(d, L, j0);
+        const Row B = finish<Stage>(d, L, rB, j0 - 1, dt, inv_dt);
+        Row C = finish<Stage>(d, L, rC, j0, dt, inv_dt);
+        Row D = finish<Stage>(d, L, rD, j0 + 1, dt, inv_dt);
+        float byC = rC.by, byD = rD.by;
+        FaceStates yB = construct<Textbook>(A.h, B.h, C.h, A.b, B.b, C.b, A.hu, B.hu, C.hu,
+                                            A.hv, B.hv, C.hv, byC, rB.by);
+        FaceStates yC = construct<Textbook>(B.h, C.h, D.h, B.b, C.b, D.b, B.hu, C.hu, D.hu,
+                                            B.hv, C.hv, D.hv, byD, byC);
+        Flux gS = face_flux<1>(yB.hp, yB.qxp, yB.qyp, yC.hm, yC.qxm, yC.qym, kappa);
+
+        float mxl = 0.f, myl = 0.f;
+        bool wetl = false;
+        for (int j = j0; j < jend; ++j) {
+            // prefetch: the row after next and the next output row's inputs
+            const Raw rN = fetch<Stage>(d, L, min(j + 3, jend + 1));
+            const Src srcN = fetch_src(d, L, min(j + 1, jend - 1));
+
+            const Row E = finish<Stage>(d, L, rE, j + 2, dt, inv_dt);
+            const float byE = rE.by;
+            const FaceStates yD = construct<Textbook>(C.h, D.h, E.h, C.b, D.b, E.b, C.hu, D.hu,
+                                                      E.hu, C.hv, D.hv, E.hv, byE, byD);
+            const Flux gN = face_flux<1>(yC.hp, yC.qxp, yC.qyp, yD.hm, yD.qxm, yD.qym, kappa);
+
+            // x pass of row j
+            const float bxw = src.bx;
+            const float bxe = from_right(bxw);
+            const FaceStates xs = construct<Textbook>(
+                from_left(C.h), C.h, from_right(C.h), from_left(C.b), C.b, from_right(C.b),
+                from_left(C.hu), C.hu, from_right(C.hu), from_left(C.hv), C.hv, from_right(C.hv),
+                bxe, bxw);
+            const Flux fE = face_flux<0>(xs.hp, xs.qxp, xs.qyp, from_right(xs.hm),
+                                         from_right(xs.qxm), from_right(xs.qym), kappa);
+            const float fw0 = from_left(fE.f0), fw1 = from_left(fE.f1), fw2 = from_left(fE.f2);
+
+            if (out_lane && act_own) {
+                const size_t qc = cell(d, L.c, j);
+                float dh = -(fE.f0 - fw0) * inv_dx;
+                float dhu = -(fE.f1 - fw1) * inv_dx;
+                float dhv = -(fE.f2 - fw2) * inv_dx;
+                dh += -(gN.f0 - gS.f0) * inv_dy;
+                dhu += -(gN.f1 - gS.f1) * inv_dy;
+                dhv += -(gN.f2 - gS.f2) * inv_dy;
+                mxl = fmaxf(mxl, fE.speed);
+                myl = fmaxf(myl, gN.speed);
+
+                const float hC = C.h;
+                const float S1 = -kG * hC * ((bxe - bxw) * inv_dx);
+                const float S2 = -kG * hC * ((byD - byC) * inv_dy);
+                float S0 = 0.f;
+                if (fl & RIFT_RAIN_AVG) S0 += rain_rate;
+                if (fl & RIFT_RAIN_GRID) S0 += src.rain;
+                if (fl & RIFT_DRAIN_AVG) {
+                    const float cap = hC * inv_dt;
+                    S0 -= (drain_rate > cap) ? cap : drain_rate;
+                }
+                if (fl & RIFT_DRAIN_GRID) {
+                    const float cap = hC * inv_dt;
+                    S0 -= (src.drain > cap) ? cap : src.drain;
+                }
+                const bool srcc = (fl & RIFT_HYDROGRAPH) && L.c >= d.si0 && L.c <= d.si1 &&
+                                  j >= d.sj0 && j <= d.sj1;
+                if (srcc) S0 += src_rate;
+                float Fnew = 0.f;
+                if (fl & RIFT_INFILTRATION) {
+                    if constexpr (Stage == 1) {
+                        S0 -= green_ampt(src.K, src.F, d.psi_dtheta, hC, dt, inv_dt);
+                    } else {
+                        // the stage-1 rate (previous dt, stored depth), then
+                        // F* = F + dF dt of Integrate_1_k (grid.cu:917-923)
+                        const float f1 = green_ampt(src.K, src.F, d.psi_dtheta, C.hraw, dtp, inv_dtp);
+                        const float Fs = src.F + f1 * dt;
+                        const float f2 = green_ampt(src.K, Fs, d.psi_dtheta, hC, dt, inv_dt);
+                        S0 -= f2;
+                        Fnew = 0.5f * (src.F + Fs + f2 * dt);
+                    }
+                }
+                dh += S0;
+                dhu += S1;
+                dhv += S2;
+                if constexpr (Stage == 1) {
+                    d.dh[qc] = dh;
+                    d.dhu[qc] = dhu;
+                    d.dhv[qc] = dhv;
+                } else {
+                    // Integrate_2_k (grid.cu:1077-1123)
+                    const float hn = 0.5f * (C.hn + C.h + dh * dt);
+                    const float hun = 0.5f * (C.hun + C.hu + dhu * dt);
+                    const float hvn = 0.5f * (C.hvn + C.hv + dhv * dt);
+                    const float hc = (hn > kEps) ? hn : 0.f;
+                    const size_t qo = qc + (L.sel[1] ? 0 : (size_t)d.soff);
+                    d.h[qo] = hc;
+                    d.hu[qo] = hun;
+                    d.hv[qo] = hvn;
+                    if (fl & RIFT_INFILTRATION) d.F[qc] = Fnew;
+                    if (fl & RIFT_SAVE_MAX) {
+                        const float fv = vel_factor(hc, kappa);
+                        const float uC = fv * hun, vC = fv * hvn;
+                        d.hmax[qc] = fmaxf(d.hmax[qc], hc);
+                        d.qmax[qc] = fmaxf(d.qmax[qc], hc * fsqrt(vC * vC + uC * uC));
+                    }
+                    if (fl & RIFT_SAVE_ARRIVAL) {
+                        if (d.twet[qc] < 0.f && hc > kEps) d.twet[qc] = tnow;
+                    }
+                    bool w = hc > kEps || srcc;
+                    if ((fl & RIFT_RAIN_AVG) && rain_rate > 0.f) w = true;
+                    if ((fl & RIFT_RAIN_GRID) && src.rain > 0.f) w = true;
+                    wetl |= w;
+                }
+            }
+            // roll the window
+            C = D;
+            D = E;
+            byC = byD;
+            byD = byE;
+            yC = yD;
+            gS = gN;
+            rE = rN;
+            src = srcN;
+        }
+
+        // per-tile reductions: speeds (both stages), wet flags (stage 2)
+        int txa, txb;
+        strip_tiles(d, s, txa, txb);
+        for (int tx = txa; tx <= txb; ++tx) {
+            const int T = rr * d.nbx + tx;
+            if (!d.act[T]) continue;   // uniform
+            const bool seg = out_lane && L.txo == tx;
+            const float m1 = wave_max(seg ? mxl : 0.f);
+            const float m2 = wave_max(seg ? myl : 0.f);
+            const bool anyw = __ballot(seg && wetl) != 0ull;
+            if (lane == 0) {
+                atomicMax(reinterpret_cast<unsigned*>(d.mx) + T, __float_as_uint(m1));
+                atomicMax(reinterpret_cast<unsigned*>(d.my) + T, __float_as_uint(m2));
+                if constexpr (Stage == 2) {
+                    d.flip[T] = 1;
+                    // one atomic per tile and unit: count down, count wet
+                    // contributors; the last contributor sees both totals
+                    const unsigned old = atomicAdd(d.pending + T, (anyw ? 1u : 0u) - (1u << 8));
+                    if ((old >> 8) == 1u) {
+                        // Integrate_2_k's flag write, by thread (0,0) of the
+                        // tile, i.e. only when the tile's origin cell is
+                        // interior (grid.cu:1160-1170)
+                        const bool acc = ((old & 0xFFu) + (anyw ? 1u : 0u)) != 0u;
+                        const int bx = T % d.nbx, by = T / d.nbx;
+                        if (bx * kTileW + 2 <= d.nx - 3 && by * kTileH + 2 <= d.ny - 3)
+                            d.wet[T] = (!(fl & RIFT_TILE_ACCEL) || acc) ? 1 : 0;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// canonical layout for the per-phase kernels and transfers: pending flips
+// applied, every tile's state gathered into h/hu/hv, selects cleared.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_apply_flips(Dev d)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < d.ntiles && d.flip[t]) {
+        d.sel[t] ^= 1;
+        d.flip[t] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_gather(Dev d, int rows)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)rows * d.P;
+    if (k >= total) return;
+    const int j = (int)(k / d.P), i = (int)(k % d.P);
+    const int ic = min(max(i, 2), d.nx - 3), jc = min(max(j, 2), d.ny - 3);
+    const int T = ((jc - 2) / kTileH) * d.nbx + (ic - 2) / kTileW;
+    if (d.sel[T]) {
+        d.h[k] = d.h0[k];
+        d.hu[k] = d.hu0[k];
+        d.hv[k] = d.hv0[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_clear_sel(Dev d)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < d.ntiles) d.sel[t] = 0;
+}
+
+void launch_grow2(const Dev& d, hipStream_t s)
+{
+    const int n = d.ntiles + d.nunits;
+    hipLaunchKernelGGL(k_grow2, dim3((n + 255) / 256), dim3(256), 0, s, d);
+}
+
+void launch_march(const Dev& d, int stage, hipStream_t s)
+{
+    int blocks = (d.nunits + 3) / 4;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    const bool tb = d.flags & RIFT_TEXTBOOK;
+    if (stage == 1) {
+        if (tb) hipLaunchKernelGGL((k_march<1, true>), dim3(blocks), dim3(256), 0, s, d);
+        else hipLaunchKernelGGL((k_march<1, false>), dim3(blocks), dim3(256), 0, s, d);
+    } else {
+        if (tb) hipLaunchKernelGGL((k_march<2, true>), dim3(blocks), dim3(256), 0, s, d);
+        else hipLaunchKernelGGL((k_march<2, false>), dim3(blocks), dim3(256), 0, s, d);
+    }
+}
+
+void launch_canonicalize(const Dev& d, hipStream_t s)
+{
+    const int tb = (d.ntiles + 255) / 256;
+    hipLaunchKernelGGL(k_apply_flips, dim3(tb), dim3(256), 0, s, d);
+    const int rows = kTileH * d.nby + 4;
+    const long total = (long)rows * d.P;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d, rows);
+    hipLaunchKernelGGL(k_clear_sel, dim3(tb), dim3(256), 0, s, d);
+}
+
+}  // namespace rift

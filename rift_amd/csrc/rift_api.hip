@@ -52,6 +52,8 @@ struct rift_grid {
     double* dsum = nullptr;       // device scratch for FP64 sums
     int grid_tiles = 0;
     bool static_loaded = false, inited = false;
+    bool scattered = false;       // fused steps left per-tile buffer selects
+    bool per_phase = false;       // rift_step uses the one-kernel-per-phase path
     bool prof = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
     std::vector<hipEvent_t> event_pool;
@@ -170,7 +172,12 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.nbx = (int)std::ceil((double)d.nx / (double)kTileW);   // grid.cu:336-337
     d.nby = (int)std::ceil((double)d.ny / (double)kTileH);
     d.ntiles = d.nbx * d.nby;
-    d.P = ((kTileW * d.nbx + 4 + 63) / 64) * 64;
+    d.nstrips = (d.nx - 4 + kStrip - 1) / kStrip;
+    d.nurows = (d.ny - 4 + kTileH - 1) / kTileH;
+    d.nunits = d.nstrips * d.nurows;
+    // the pitch covers the 20-column window of the last tile and the 64
+    // lanes of the last strip
+    d.P = ((std::max(kTileW * d.nbx + 4, kStrip * d.nstrips + 4) + 63) / 64) * 64;
     g->R = kTileH * d.nby + 4;
     g->plane = (size_t)d.P * (size_t)g->R;
     d.dx = desc->dx;
@@ -195,7 +202,20 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     const unsigned f = desc->flags;
     bool ok = true;
     auto P = [&](float*& p) { p = alloc_plane(g); ok = ok && p; };
-    P(d.h); P(d.hu); P(d.hv); P(d.h0); P(d.hu0); P(d.hv0);
+    // the two state buffers are six consecutive planes, so a tile's buffer
+    // select is one element offset (d.soff) added to the h/hu/hv indices
+    {
+        void* p = nullptr;
+        ok = hipMalloc(&p, 6 * g->plane * sizeof(float)) == hipSuccess &&
+             hipMemset(p, 0, 6 * g->plane * sizeof(float)) == hipSuccess;
+        if (ok) {
+            g->allocs.push_back(p);
+            float* b = static_cast<float*>(p);
+            d.h = b; d.hu = b + g->plane; d.hv = b + 2 * g->plane;
+            d.h0 = b + 3 * g->plane; d.hu0 = b + 4 * g->plane; d.hv0 = b + 5 * g->plane;
+            d.soff = (long long)(3 * g->plane);
+        }
+    }
     P(d.dh); P(d.dhu); P(d.dhv); P(d.G); P(d.n); P(d.BC); P(d.BX); P(d.BY);
     if (f & RIFT_INFILTRATION) { P(d.F); P(d.F0); P(d.dF); P(d.K); }
     if (f & RIFT_SAVE_MAX) { P(d.hmax); P(d.qmax); P(d.tpeak); P(d.tdry); }
@@ -208,10 +228,15 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.list = alloc_bytes<int>(g, d.ntiles);
     d.mx = alloc_bytes<float>(g, d.ntiles);
     d.my = alloc_bytes<float>(g, d.ntiles);
+    d.sel = alloc_bytes<unsigned char>(g, d.ntiles);
+    d.flip = alloc_bytes<unsigned char>(g, d.ntiles);
+    d.pending = alloc_bytes<unsigned>(g, d.ntiles);
+    d.ulist = alloc_bytes<int>(g, d.nunits);
     d.sc = alloc_bytes<Scalars>(g, 1);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
-    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.sc && d.ser && g->dsum;
+    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.sc && d.ser && g->dsum && d.sel &&
+         d.flip && d.pending && d.ulist;
     if (!ok) {
         rift_destroy(g);
         return fail(RIFT_E_HIP, "rift_create: device allocation failed");
@@ -268,6 +293,12 @@ int rift_upload_static(rift_grid* g, const double* BX, const double* BY, const d
     std::vector<double> zero((size_t)nx * ny, 0.0);
     g->n_host.assign(n ? n : zero.data(), (n ? n : zero.data()) + (size_t)nx * ny);
     if ((rc = put_field(g, d.n, g->n_host.data(), nx, ny, nx))) return rc;
+    // a Manning field that is constant over the interior is read as a scalar
+    d.n_uniform = 1;
+    d.n_const = (float)g->n_host[(size_t)2 * nx + 2];
+    for (int j = 2; j <= ny - 3 && d.n_uniform; ++j)
+        for (int i = 2; i <= nx - 3; ++i)
+            if ((float)g->n_host[(size_t)j * nx + i] != d.n_const) { d.n_uniform = 0; break; }
     if (d.K) {
         if (!K) return fail(RIFT_E_ARG, "rift_upload_static: K required with infiltration");
         if ((rc = put_field(g, d.K, K, nx, ny, nx))) return rc;
@@ -308,6 +339,10 @@ int rift_init(rift_grid* g, double t0)
     HIPCHK(hipMemcpyAsync(d.act, wet.data(), d.ntiles, hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipMemsetAsync(d.mx, 0, d.ntiles * sizeof(float), g->stream));
     HIPCHK(hipMemsetAsync(d.my, 0, d.ntiles * sizeof(float), g->stream));
+    HIPCHK(hipMemsetAsync(d.sel, 0, d.ntiles, g->stream));
+    HIPCHK(hipMemsetAsync(d.flip, 0, d.ntiles, g->stream));
+    HIPCHK(hipMemsetAsync(d.pending, 0, d.ntiles * sizeof(unsigned), g->stream));
+    g->scattered = false;
     Scalars sc{};
     HIPCHK(hipMemcpyAsync(&sc, d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -316,6 +351,7 @@ int rift_init(rift_grid* g, double t0)
     sc = Scalars{};
     sc.t = t0;
     sc.dt = (double)0.0000001f;   // simulator.cu:35
+    sc.dt_prev = sc.dt;
     sc.hydro_external = ext;
     sc.hydro_value = hv;
     HIPCHK(hipMemcpyAsync(d.sc, &sc, sizeof(Scalars), hipMemcpyHostToDevice, g->stream));
@@ -410,10 +446,25 @@ int rift_copy_map(rift_grid* g, int which, int dst, int src)
     return RIFT_OK;
 }
 
+// Per-phase kernels and transfers work on the canonical layout: every
+// tile's state in h/hu/hv with its ghost cells filled.
+static int ensure_canonical(rift_grid* g)
+{
+    if (!g->scattered) return RIFT_OK;
+    launch_canonicalize(g->d, g->stream);
+    launch_boundaries(g->d, g->stream);
+    int rc = check_launch("canonicalize");
+    if (rc) return rc;
+    g->scattered = false;
+    return RIFT_OK;
+}
+
 #define NEED_INIT(g)                                                        \
     do {                                                                    \
         if (!(g)) return fail(RIFT_E_ARG, "null handle");                   \
         if (!(g)->inited) return fail(RIFT_E_STATE, "rift_init not called"); \
+        int rc_ = ensure_canonical(g);                                      \
+        if (rc_) return rc_;                                                \
     } while (0)
 
 int rift_update_source(rift_grid* g)
@@ -472,11 +523,48 @@ int rift_apply_boundaries(rift_grid* g)
     return check_launch("boundaries");
 }
 
+// The fused step: forcing, tile/unit compaction, stage-1 march, dt,
+// stage-2 march (Integrate_1 + flux + Heun + diagnostics in one pass).
+static int step_fused(rift_grid* g)
+{
+    Dev& d = g->d;
+    if (d.flags & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG)) launch_begin_step(d, g->stream);
+    {
+        ProfScope ps(g, 4);
+        if (d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(d, g->stream);
+        launch_reset_counters(d, g->stream);
+        launch_grow2(d, g->stream);
+    }
+    {
+        ProfScope ps(g, 0);
+        launch_march(d, 1, g->stream);
+    }
+    {
+        ProfScope ps(g, 5);
+        launch_timestep(d, g->stream);
+    }
+    {
+        ProfScope ps(g, 1);
+        launch_march(d, 2, g->stream);
+    }
+    g->scattered = true;
+    return check_launch("fused step");
+}
+
 int rift_step(rift_grid* g, int nsteps)
 {
-    NEED_INIT(g);
+    if (!g) return fail(RIFT_E_ARG, "null handle");
+    if (!g->inited) return fail(RIFT_E_STATE, "rift_init not called");
     if (nsteps < 0) return fail(RIFT_E_ARG, "rift_step: negative step count");
     const bool euler = g->d.flags & RIFT_EULER;
+    if (!euler && !g->per_phase) {
+        for (int s = 0; s < nsteps; ++s) {
+            int rc = step_fused(g);
+            if (rc) return rc;
+        }
+        return RIFT_OK;
+    }
+    NEED_INIT(g);
     for (int s = 0; s < nsteps; ++s) {
         int rc;
         if (g->d.flags & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG))
@@ -497,6 +585,19 @@ int rift_step(rift_grid* g, int nsteps)
         }
     }
     return RIFT_OK;
+}
+
+int rift_set_option(rift_grid* g, int option, int value)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_set_option: null handle");
+    switch (option) {
+    case RIFT_OPT_STEP_PATH:
+        if (value != 0 && value != 1) return fail(RIFT_E_ARG, "rift_set_option: step path is 0 or 1");
+        g->per_phase = value == 1;
+        return RIFT_OK;
+    default:
+        return fail(RIFT_E_ARG, "rift_set_option: unknown option");
+    }
 }
 
 int rift_sync(rift_grid* g)
@@ -556,6 +657,10 @@ static float* field_plane(rift_grid* g, int field)
 int rift_download(rift_grid* g, int field, double* host)
 {
     if (!g || !host) return fail(RIFT_E_ARG, "rift_download: null argument");
+    {
+        int rc0 = ensure_canonical(g);
+        if (rc0) return rc0;
+    }
     Dev& d = g->d;
     const int nx = d.nx, ny = d.ny;
     std::vector<float> a, b, c;
@@ -595,6 +700,10 @@ int rift_download(rift_grid* g, int field, double* host)
 int rift_upload(rift_grid* g, int field, const double* host)
 {
     if (!g || !host) return fail(RIFT_E_ARG, "rift_upload: null argument");
+    {
+        int rc0 = ensure_canonical(g);
+        if (rc0) return rc0;
+    }
     Dev& d = g->d;
     const int nx = d.nx, ny = d.ny;
     if (field == RIFT_W) {
@@ -666,6 +775,10 @@ int rift_get_rates(rift_grid* g, float* out3)
 int rift_sum_interior(rift_grid* g, int field, double* out)
 {
     if (!g || !out) return fail(RIFT_E_ARG, "rift_sum_interior: null argument");
+    {
+        int rc0 = ensure_canonical(g);
+        if (rc0) return rc0;
+    }
     float* p = field_plane(g, field);
     if (!p) return fail(RIFT_E_ARG, "rift_sum_interior: field not available");
     HIPCHK(hipMemsetAsync(g->dsum, 0, sizeof(double), g->stream));

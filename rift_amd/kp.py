@@ -72,6 +72,7 @@ SIGNATURES = [
     ("rift_integrate_1", C.c_int, [_VP]),
     ("rift_integrate_2", C.c_int, [_VP]),
     ("rift_apply_boundaries", C.c_int, [_VP]),
+    ("rift_set_option", C.c_int, [_VP, C.c_int, C.c_int]),
     ("rift_sync", C.c_int, [_VP]),
     ("rift_get_time", C.c_int, [_VP, _D, _D]),
     ("rift_set_time", C.c_int, [_VP, C.c_double, C.c_double]),
@@ -215,6 +216,10 @@ class Grid:
     def phase(self, name):
         _check(getattr(load(), "rift_" + name)(self.h), "rift_" + name)
 
+    def set_path(self, per_phase):
+        """True: one kernel per reference phase; False: the fused step."""
+        _check(load().rift_set_option(self.h, 1, 1 if per_phase else 0), "rift_set_option")
+
     def sync(self):
         _check(load().rift_sync(self.h), "rift_sync")
 
@@ -302,7 +307,7 @@ class Grid:
 
 
 def grid_from_rasters(dem, dx, *, h0=None, n=None, n_const=0.0, K=None, psi=0.0, dtheta=0.0,
-                      courant=0.25, flags=TILE_ACCEL, t0=0.0, device=0):
+                      courant=0.25, flags=TILE_ACCEL, t0=0.0, device=0, per_phase=False):
     """Build, upload and initialise a Grid from corner rasters (row 0 = south)."""
     if h0 is not None:
         flags |= H_INIT
@@ -313,4 +318,6 @@ def grid_from_rasters(dem, dx, *, h0=None, n=None, n_const=0.0, K=None, psi=0.0,
              device=device)
     g.upload_static(geo["BX"], geo["BY"], geo["n"], geo["K"], geo["h0"])
     g.init(t0)
+    if per_phase:
+        g.set_path(True)
     return g
