@@ -88,6 +88,36 @@ def cpu_baseline(cfg, g, budget_s=15.0, max_steps=60):
                       f"per-cell update (oracle/liboracle32.so), {el:.1f} s, {threads} threads"}
 
 
+def _active_interior_cells(nx, ny, nbx, act):
+    """Interior cells of the tiles in an active list (reference encoding)."""
+    total = 0
+    for t in act:
+        if t < 0:
+            continue
+        bx, by = t % nbx, t // nbx
+        i0, j0 = bx * 16 + 2, by * 14 + 2
+        ni = min(i0 + 16, nx - 2) - i0
+        nj = min(j0 + 14, ny - 2) - j0
+        if ni > 0 and nj > 0:
+            total += ni * nj
+    return total
+
+
+def calibration_copy(nbytes=256 << 20):
+    """One hipMemcpyDtoD of nbytes (runs as the runtime's copy kernel)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0     # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +125,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=160)
     ap.add_argument("--cells", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (a known-size reference "
+                         "for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -184,6 +217,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(cfg, g)
     else:
         line["cpu_baseline"] = None
+    if args.calibrate:
+        line["calibration_copy_bytes"] = calibration_copy()
     g.close()
     print(json.dumps(line))
 

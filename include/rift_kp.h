@@ -161,6 +161,9 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
  * (default), 1 = one kernel per reference phase (Grow, ComputeFluxes,
  * Integrate_1, ApplyBoundaries, ...); both give the same results. */
 #define RIFT_OPT_STEP_PATH 1
+/* RIFT_OPT_GRAPHS: 1 (default) = replay the fused step from captured HIP
+ * graphs (16 steps per launch), 0 = launch its kernels one by one. */
+#define RIFT_OPT_GRAPHS 2
 int rift_set_option(rift_grid* g, int option, int value);
 
 /* Wait for all queued work of the handle. */

@@ -53,6 +53,9 @@ __global__ void k_begin_step(Dev d)
 {
     if (threadIdx.x != 0) return;
     Scalars* sc = d.sc;
+    sc->n_active = 0;
+    sc->n_units = 0;
+    sc->wet_cells = 0ull;
     const unsigned f = d.flags;
     if (!(f & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG))) return;
     const float tf = (float)sc->t;
@@ -341,20 +344,9 @@ __global__ __launch_bounds__(256) void k_timestep(Dev d)
     if (last && threadIdx.x == 0) {
         Scalars* sc = d.sc;
         __threadfence();
-        double max_x = (double)__uint_as_float(atomicAdd(&sc->mx_bits, 0u));
-        double max_y = (double)__uint_as_float(atomicAdd(&sc->my_bits, 0u));
-        double c_x = d.dx / (double)fmaxf((float)(max_x / d.courant), (float)d.kappa_d);
-        double c_y = d.dy / (double)fmaxf((float)(max_y / d.courant), (float)d.kappa_d);
-        double dt = (c_x < c_y) ? c_x : c_y;
-        if (sc->t == 0.0) dt = (double)0.000001f;
-        sc->dt_prev = sc->dt;
-        sc->dt = dt;
-        sc->t += dt;
-        // FP64 mass accounting of the scalar inflows applied this step
-        double cells = (double)sc->wet_cells;
-        if (d.flags & RIFT_RAIN_AVG) sc->v_rain += (double)sc->rain_rate * dt * cells;
-        if (d.flags & RIFT_HYDROGRAPH)
-            sc->v_src += (double)sc->src_cell_rate * dt * (double)d.nsrc;
+        const float vx_all = __uint_as_float(atomicAdd(&sc->mx_bits, 0u));
+        const float vy_all = __uint_as_float(atomicAdd(&sc->my_bits, 0u));
+        finish_timestep(d, vx_all, vy_all);
         sc->mx_bits = 0u;
         sc->my_bits = 0u;
         sc->ticket = 0u;
@@ -439,9 +431,9 @@ __global__ __launch_bounds__(256) void k_integrate1(Dev d)
                     }
                 }
                 float hn = h + dh * dt;
-                float den = 1.f - dt * d.G[q];
-                hu = (hu + d.dhu[q] * dt) / den;
-                hv = (hv + d.dhv[q] * dt) / den;
+                const float rden = frcp(1.f - dt * d.G[q]);
+                hu = (hu + d.dhu[q] * dt) * rden;
+                hv = (hv + d.dhv[q] * dt) * rden;
                 float hc = (hn > kEps) ? hn : 0.f;
                 d.h[q] = hc; d.hu[q] = hu; d.hv[q] = hv;
                 if (d.flags & RIFT_INFILTRATION) {

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/rift_kp.h"
+
 namespace rift {
 
 constexpr int kMaxSeries = 4096;   // points per scalar series
@@ -61,7 +63,8 @@ struct Dev {
     long long soff;              // h0 - h (elements): offset of the second buffer
     int n_uniform;
     float n_const;
-    float *mx, *my;
+    float *mx, *my;              // per-tile maxima of the last flux pass
+    float *mx1, *my1;            // fused step: stage-1 maxima of this step
     Scalars* sc;
     SeriesTable* ser;            // [3]: hyetograph, drain, hydrograph
     int P, nx, ny, nbx, nby, ntiles;
@@ -75,6 +78,24 @@ struct Dev {
 __host__ __device__ inline size_t cell(const Dev& d, int i, int j)
 {
     return (size_t)j * (size_t)d.P + (size_t)i;
+}
+
+// ComputeTimestep's arithmetic (simulator.cu:549-561) on one lane; also the
+// FP64 accounting of the scalar inflows of the step.
+__device__ inline void finish_timestep(const Dev& d, float vx, float vy)
+{
+    Scalars* sc = d.sc;
+    const double max_x = (double)vx, max_y = (double)vy;
+    const double c_x = d.dx / (double)fmaxf((float)(max_x / d.courant), (float)d.kappa_d);
+    const double c_y = d.dy / (double)fmaxf((float)(max_y / d.courant), (float)d.kappa_d);
+    double dt = (c_x < c_y) ? c_x : c_y;
+    if (sc->t == 0.0) dt = (double)0.000001f;
+    sc->dt_prev = sc->dt;
+    sc->dt = dt;
+    sc->t += dt;
+    const double cells = (double)sc->wet_cells;
+    if (d.flags & RIFT_RAIN_AVG) sc->v_rain += (double)sc->rain_rate * dt * cells;
+    if (d.flags & RIFT_HYDROGRAPH) sc->v_src += (double)sc->src_cell_rate * dt * (double)d.nsrc;
 }
 
 // Launchers (kp_kernels.hip).  All run on `s` and read t/dt/counts from d.sc.
@@ -94,6 +115,7 @@ void launch_sum_interior(const Dev& d, const float* x, int mode, double* out, hi
 // columns (lanes 2..61) by one 14-row tile row.
 constexpr int kStrip = 60;
 void launch_grow2(const Dev& d, hipStream_t s);
+void launch_timestep2(const Dev& d, hipStream_t s);
 void launch_march(const Dev& d, int stage, hipStream_t s);
 void launch_canonicalize(const Dev& d, hipStream_t s);
 

@@ -93,6 +93,8 @@ __global__ __launch_bounds__(256) void k_grow2(Dev d)
             }
             d.mx[t] = 0.f;
             d.my[t] = 0.f;
+            d.mx1[t] = 0.f;
+            d.my1[t] = 0.f;
             // contributor count in bits 8.., wet contributors in bits 0..7
             d.pending[t] = units << 8;
         }
@@ -214,9 +216,9 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
             if ((d.flags & RIFT_SURGE_GRID) && raw.sg != d.nodata && raw.sg > 0.f)
                 dh = (raw.sg - (hcl + dh * dt)) * inv_dt;
             const float hnew = hcl + dh * dt;
-            const float den = 1.f - dt * G;
-            hu = (hu + raw.dhu * dt) / den;
-            hv = (hv + raw.dhv * dt) / den;
+            const float rden = frcp(1.f - dt * G);
+            hu = (hu + raw.dhu * dt) * rden;
+            hv = (hv + raw.dhv * dt) * rden;
             h = (hnew > kEps) ? hnew : 0.f;
         }
     }
@@ -249,8 +251,30 @@ __global__ __launch_bounds__(256) void k_march(Dev d)
     const float tnow = (float)d.sc->t;
     const float inv_dx = d.inv_dx, inv_dy = d.inv_dy;
 
-    for (int k = wave; k < nunits; k += nwaves) {
-        const int u = d.ulist[k];
+    // Work order.  Workgroup b runs on XCD b % 8, each XCD with its own L2.
+    // When every unit is active, XCD x takes the x-th eighth of the units
+    // in row-major order, so the units that share halo rows and boundary
+    // cache lines with a unit (its vertical and horizontal neighbours) run
+    // on the same XCD at about the same time.  Otherwise the compacted list
+    // is walked grid-stride.
+    const bool dense = nunits == d.nunits;
+    int kbeg, kend, kstep;
+    if (dense) {
+        // bands = min(8, workgroups): every band has at least one workgroup
+        const int nb = min((int)gridDim.x, 8);
+        const int x = (int)blockIdx.x % nb;
+        const int nwg_x = ((int)gridDim.x - x + nb - 1) / nb;
+        kbeg = (int)((long long)d.nunits * x / nb) + ((int)blockIdx.x / nb) * 4 + (threadIdx.x >> 6);
+        kend = (int)((long long)d.nunits * (x + 1) / nb);
+        kstep = nwg_x * 4;
+    } else {
+        kbeg = wave;
+        kend = nunits;
+        kstep = nwaves;
+    }
+    (void)nwaves;
+    for (int k = kbeg; k < kend; k += kstep) {
+        const int u = dense ? k : d.ulist[k];
         const int s = u % d.nstrips, rr = u / d.nstrips;
         Lane L;
         L.c = kStrip * s + lane;
@@ -395,7 +419,11 @@ __global__ __launch_bounds__(256) void k_march(Dev d)
             src = srcN;
         }
 
-        // per-tile reductions: speeds (both stages), wet flags (stage 2)
+        // per-tile speed maxima: stage 1 into mx1/my1 (this step's dt),
+        // stage 2 into mx/my (what later steps see while the tile is
+        // inactive); stage 2 also forms the tile wet flags
+        float* const tmx = (Stage == 1) ? d.mx1 : d.mx;
+        float* const tmy = (Stage == 1) ? d.my1 : d.my;
         int txa, txb;
         strip_tiles(d, s, txa, txb);
         for (int tx = txa; tx <= txb; ++tx) {
@@ -406,8 +434,8 @@ __global__ __launch_bounds__(256) void k_march(Dev d)
             const float m2 = wave_max(seg ? myl : 0.f);
             const bool anyw = __ballot(seg && wetl) != 0ull;
             if (lane == 0) {
-                atomicMax(reinterpret_cast<unsigned*>(d.mx) + T, __float_as_uint(m1));
-                atomicMax(reinterpret_cast<unsigned*>(d.my) + T, __float_as_uint(m2));
+                atomicMax(reinterpret_cast<unsigned*>(tmx) + T, __float_as_uint(m1));
+                atomicMax(reinterpret_cast<unsigned*>(tmy) + T, __float_as_uint(m2));
                 if constexpr (Stage == 2) {
                     d.flip[T] = 1;
                     // one atomic per tile and unit: count down, count wet
@@ -460,6 +488,41 @@ __global__ __launch_bounds__(256) void k_clear_sel(Dev d)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < d.ntiles) d.sel[t] = 0;
+}
+
+// dt of the fused step: max over every tile of the stage-1 maxima (active
+// tiles) or the stage-2 maxima of the last step a tile was active (others),
+// in one workgroup, then ComputeTimestep's arithmetic.
+__global__ __launch_bounds__(1024) void k_timestep2(Dev d)
+{
+    __shared__ float red[2][16];
+    float vx = 0.f, vy = 0.f;
+    for (int t = threadIdx.x; t < d.ntiles; t += 1024) {
+        const bool a = d.act[t] != 0;
+        vx = fmaxf(vx, a ? d.mx1[t] : d.mx[t]);
+        vy = fmaxf(vy, a ? d.my1[t] : d.my[t]);
+    }
+    vx = wave_max(vx);
+    vy = wave_max(vy);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = vx;
+        red[1][threadIdx.x >> 6] = vy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; ++w) {
+            vx = fmaxf(vx, red[0][w]);
+            vy = fmaxf(vy, red[1][w]);
+        }
+        vx = fmaxf(vx, red[0][0]);
+        vy = fmaxf(vy, red[1][0]);
+        finish_timestep(d, vx, vy);
+    }
+}
+
+void launch_timestep2(const Dev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_timestep2, dim3(1), dim3(1024), 0, s, d);
 }
 
 void launch_grow2(const Dev& d, hipStream_t s)

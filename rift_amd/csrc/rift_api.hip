@@ -54,10 +54,21 @@ struct rift_grid {
     bool static_loaded = false, inited = false;
     bool scattered = false;       // fused steps left per-tile buffer selects
     bool per_phase = false;       // rift_step uses the one-kernel-per-phase path
+    // captured fused steps (1 and kGraphSteps steps); the kernels take the
+    // handle's Dev by value, so any change of it invalidates the graphs
+    hipGraphExec_t graph1 = nullptr, graphN = nullptr;
+    bool use_graphs = true;
     bool prof = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
     std::vector<hipEvent_t> event_pool;
 };
+
+void drop_graphs(rift_grid* g)
+{
+    if (g->graph1) (void)hipGraphExecDestroy(g->graph1);
+    if (g->graphN) (void)hipGraphExecDestroy(g->graphN);
+    g->graph1 = g->graphN = nullptr;
+}
 
 namespace {
 
@@ -228,6 +239,8 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.list = alloc_bytes<int>(g, d.ntiles);
     d.mx = alloc_bytes<float>(g, d.ntiles);
     d.my = alloc_bytes<float>(g, d.ntiles);
+    d.mx1 = alloc_bytes<float>(g, d.ntiles);
+    d.my1 = alloc_bytes<float>(g, d.ntiles);
     d.sel = alloc_bytes<unsigned char>(g, d.ntiles);
     d.flip = alloc_bytes<unsigned char>(g, d.ntiles);
     d.pending = alloc_bytes<unsigned>(g, d.ntiles);
@@ -235,7 +248,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.sc = alloc_bytes<Scalars>(g, 1);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
-    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.sc && d.ser && g->dsum && d.sel &&
+    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.ser && g->dsum && d.sel &&
          d.flip && d.pending && d.ulist;
     if (!ok) {
         rift_destroy(g);
@@ -250,6 +263,7 @@ int rift_destroy(rift_grid* g)
 {
     if (!g) return RIFT_OK;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
+    drop_graphs(g);
     for (auto& pe : g->prof_events) {
         (void)hipEventDestroy(pe.second.first);
         (void)hipEventDestroy(pe.second.second);
@@ -306,6 +320,7 @@ int rift_upload_static(rift_grid* g, const double* BX, const double* BY, const d
     // the initial depth is held in h0 until rift_init
     if ((rc = put_field(g, d.h0, h0 ? h0 : zero.data(), nx, ny, nx))) return rc;
     g->static_loaded = true;
+    drop_graphs(g);
     return RIFT_OK;
 }
 
@@ -391,7 +406,8 @@ int rift_set_source_cells(rift_grid* g, const int* idx, long n)
         d.si0 = d.sj0 = 1;
         d.si1 = d.sj1 = 0;
         d.nsrc = 0;
-        return RIFT_OK;
+        drop_graphs(g);
+    return RIFT_OK;
     }
     int i0 = 1 << 30, i1 = -1, j0 = 1 << 30, j1 = -1;
     std::vector<long> seen;
@@ -409,6 +425,7 @@ int rift_set_source_cells(rift_grid* g, const int* idx, long n)
         return fail(RIFT_E_UNSUPPORTED, "rift_set_source_cells: cells must form a rectangle");
     d.si0 = i0; d.si1 = i1; d.sj0 = j0; d.sj1 = j1;
     d.nsrc = n;
+    drop_graphs(g);
     return RIFT_OK;
 }
 
@@ -523,16 +540,37 @@ int rift_apply_boundaries(rift_grid* g)
     return check_launch("boundaries");
 }
 
+constexpr int kGraphSteps = 16;
+
+static int step_fused(rift_grid* g);
+
+static int capture_steps(rift_grid* g, int n, hipGraphExec_t* out)
+{
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = RIFT_OK;
+    for (int k = 0; k < n && rc == RIFT_OK; ++k) rc = step_fused(g);
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(RIFT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(RIFT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return RIFT_OK;
+}
+
 // The fused step: forcing, tile/unit compaction, stage-1 march, dt,
 // stage-2 march (Integrate_1 + flux + Heun + diagnostics in one pass).
 static int step_fused(rift_grid* g)
 {
     Dev& d = g->d;
-    if (d.flags & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG)) launch_begin_step(d, g->stream);
+    launch_begin_step(d, g->stream);   // forcing + counter reset
     {
         ProfScope ps(g, 4);
         if (d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(d, g->stream);
-        launch_reset_counters(d, g->stream);
         launch_grow2(d, g->stream);
     }
     {
@@ -541,7 +579,7 @@ static int step_fused(rift_grid* g)
     }
     {
         ProfScope ps(g, 5);
-        launch_timestep(d, g->stream);
+        launch_timestep2(d, g->stream);
     }
     {
         ProfScope ps(g, 1);
@@ -558,10 +596,21 @@ int rift_step(rift_grid* g, int nsteps)
     if (nsteps < 0) return fail(RIFT_E_ARG, "rift_step: negative step count");
     const bool euler = g->d.flags & RIFT_EULER;
     if (!euler && !g->per_phase) {
-        for (int s = 0; s < nsteps; ++s) {
-            int rc = step_fused(g);
-            if (rc) return rc;
+        if (g->prof || !g->use_graphs) {
+            for (int s = 0; s < nsteps; ++s) {
+                int rc = step_fused(g);
+                if (rc) return rc;
+            }
+            return RIFT_OK;
         }
+        int rc;
+        int left = nsteps;
+        if (left >= kGraphSteps && !g->graphN && (rc = capture_steps(g, kGraphSteps, &g->graphN)))
+            return rc;
+        for (; left >= kGraphSteps; left -= kGraphSteps) HIPCHK(hipGraphLaunch(g->graphN, g->stream));
+        if (left > 0 && !g->graph1 && (rc = capture_steps(g, 1, &g->graph1))) return rc;
+        for (; left > 0; --left) HIPCHK(hipGraphLaunch(g->graph1, g->stream));
+        g->scattered = true;
         return RIFT_OK;
     }
     NEED_INIT(g);
@@ -594,6 +643,9 @@ int rift_set_option(rift_grid* g, int option, int value)
     case RIFT_OPT_STEP_PATH:
         if (value != 0 && value != 1) return fail(RIFT_E_ARG, "rift_set_option: step path is 0 or 1");
         g->per_phase = value == 1;
+        return RIFT_OK;
+    case RIFT_OPT_GRAPHS:
+        g->use_graphs = value != 0;
         return RIFT_OK;
     default:
         return fail(RIFT_E_ARG, "rift_set_option: unknown option");

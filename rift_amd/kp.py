@@ -220,6 +220,10 @@ class Grid:
         """True: one kernel per reference phase; False: the fused step."""
         _check(load().rift_set_option(self.h, 1, 1 if per_phase else 0), "rift_set_option")
 
+    def set_graphs(self, on):
+        """Replay the fused step from captured HIP graphs (default on)."""
+        _check(load().rift_set_option(self.h, 2, 1 if on else 0), "rift_set_option")
+
     def sync(self):
         _check(load().rift_sync(self.h), "rift_sync")
 
