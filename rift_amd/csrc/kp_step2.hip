@@ -149,13 +149,19 @@ struct Src {                 // own-cell inputs of an output row
 struct Lane {
     int c, sc, txo, r;
     bool gx;
-    unsigned char sel[3], act[3];
+    bool sel0, sel1, sel2;   // buffer selects of tile rows r-1, r, r+1
+    bool act0, act1, act2;   // activity of tile rows r-1, r, r+1
 };
 
 __device__ __forceinline__ int tile_row_slot(const Lane& L, int sr)
 {
     const int tr = (sr - 2) / kTileH;
-    return tr - L.r + 1;     // 0, 1 or 2
+    return tr - L.r + 1;     // 0, 1 or 2 (uniform across the wave)
+}
+
+__device__ __forceinline__ bool pick(int slot, bool a, bool b, bool c)
+{
+    return slot == 0 ? a : (slot == 1 ? b : c);
 }
 
 template <int Stage>
@@ -164,7 +170,7 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const Lane& L, int j)
     const int sr = min(max(j, 2), d.ny - 3);
     const int slot = tile_row_slot(L, sr);
     const size_t q = cell(d, L.sc, sr);
-    const size_t qs = q + (L.sel[slot] ? (size_t)d.soff : 0);
+    const size_t qs = q + (pick(slot, L.sel0, L.sel1, L.sel2) ? (size_t)d.soff : 0);
     Raw r;
     r.h = d.h[qs];
     r.hu = d.hu[qs];
@@ -207,7 +213,7 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
         r.hn = (h > kEps) ? h : 0.f;
         r.hun = hu;
         r.hvn = hv;
-        if (L.act[tile_row_slot(L, sr)]) {
+        if (pick(tile_row_slot(L, sr), L.act0, L.act1, L.act2)) {
             // Integrate_1_k (grid.cu:856-922) of this cell from U^n and dU;
             // G is recomputed from U^n exactly as the stage-1 flux forms it
             const float G = manning_g(raw.n, hu, hv, h, d.kappa);
@@ -235,8 +241,10 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
     return r;
 }
 
+// Stage 2 holds two row windows (U* and U^n); four waves per SIMD keep it
+// under 128 VGPRs without spilling.
 template <int Stage, bool Textbook>
-__global__ __launch_bounds__(256) void k_march(Dev d)
+__global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
 {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -282,14 +290,14 @@ __global__ __launch_bounds__(256) void k_march(Dev d)
         L.gx = (L.c < 2) || (L.c > d.nx - 3);
         L.txo = (L.sc - 2) / kTileW;
         L.r = rr;
-        for (int q = 0; q < 3; ++q) {
-            const int tr = min(max(rr - 1 + q, 0), d.nby - 1);
-            const int T = tr * d.nbx + L.txo;
-            L.sel[q] = d.sel[T];
-            L.act[q] = d.act[T];
+        {
+            const int Tm = max(rr - 1, 0) * d.nbx + L.txo, T0 = rr * d.nbx + L.txo;
+            const int Tp = min(rr + 1, d.nby - 1) * d.nbx + L.txo;
+            L.sel0 = d.sel[Tm] != 0; L.sel1 = d.sel[T0] != 0; L.sel2 = d.sel[Tp] != 0;
+            L.act0 = d.act[Tm] != 0; L.act1 = d.act[T0] != 0; L.act2 = d.act[Tp] != 0;
         }
         const bool out_lane = lane >= 2 && lane <= kStrip + 1 && L.c <= d.nx - 3;
-        const bool act_own = L.act[1] != 0;
+        const bool act_own = L.act1;
         const int j0 = rr * kTileH + 2;
         const int jend = min(j0 + kTileH, d.ny - 2);
 
@@ -388,7 +396,7 @@ __global__ __launch_bounds__(256) void k_march(Dev d)
                     const float hun = 0.5f * (C.hun + C.hu + dhu * dt);
                     const float hvn = 0.5f * (C.hvn + C.hv + dhv * dt);
                     const float hc = (hn > kEps) ? hn : 0.f;
-                    const size_t qo = qc + (L.sel[1] ? 0 : (size_t)d.soff);
+                    const size_t qo = qc + (L.sel1 ? 0 : (size_t)d.soff);
                     d.h[qo] = hc;
                     d.hu[qo] = hun;
                     d.hv[qo] = hvn;
