@@ -29,6 +29,77 @@ int rift_geometry(int b_nx, int b_ny, const double* dem, const double* h0c, cons
                   double n_const, const double* Kc, double* BX, double* BY, double* h0,
                   double* n, double* K);
 
+/* ---- File formats ------------------------------------------------------
+ * CSV series, Source::ReadSource (source.cc:11-63): `hours,value` lines, '#'
+ * comment lines, times converted to seconds in float, a zero-rate point
+ * appended when the last rate is nonzero.  Writes min(n, cap) points and
+ * returns the count in *n (call with cap 0 to size the buffers). */
+int rift_read_series(const char* path, float* time_s, float* rate, int cap, int* n);
+/* ESRI-ASCII raster, ReadGrid / SetOriginalGrid (io.cc:260-329): header
+ * (ncols, nrows, xllcorner, yllcorner, cellsize, NODATA_value) then rows north
+ * first.  data receives nrows x ncols doubles with row 0 = south (pass NULL to
+ * read the header only); hdr4 = {xll, yll, cellsize, nodata}.  NODATA values
+ * are an error unless allow_nodata, then they read as -9999. */
+int rift_read_raster(const char* path, int allow_nodata, int* ncols, int* nrows, double* hdr4,
+                     double* data, long cap);
+/* writeGrid (io.cc:331-361): the interior of an internal-grid field (nx x ny,
+ * row 0 = south; interior = 2 ghost cells off each side), header from the
+ * internal lower-left corner and the raster cell size, 15 significant
+ * digits, rows north first. */
+int rift_write_grid(const char* path, const double* field, int nx, int ny, double h_xll,
+                    double h_yll, double cellsize);
+
+/* ---- Keyword configuration (ConfigFile, config.h:48-209) ---------------
+ * `key = value` lines, `;` comments, unique keys.  Typed reads use stream
+ * extraction as the reference's Convert::string_to_T does (config.h:34-44):
+ * a string is the first word, a bool is 0/1.  Errors (missing file, bad
+ * line, duplicate key, bad value, missing key) return RIFT_E_ARG with the
+ * message in rift_last_error() instead of waiting on stdin and exiting. */
+typedef struct rift_config rift_config;
+int rift_config_open(const char* path, rift_config** out);
+void rift_config_close(rift_config* c);
+/* keyExists (config.h:192-195): 1 or 0. */
+int rift_config_has(const rift_config* c, const char* key);
+/* keyIsNumber (config.h:176-190): 1 when the first word holds only digits
+ * and '.', else 0; RIFT_E_ARG for a missing key. */
+int rift_config_is_number(const rift_config* c, const char* key, int* out);
+/* getValueOfKey<std::string>: copies at most cap-1 bytes plus NUL; *len
+ * (optional) receives the full length. */
+int rift_config_string(const rift_config* c, const char* key, char* buf, int cap, int* len);
+int rift_config_double(const rift_config* c, const char* key, double* out);
+int rift_config_bool(const rift_config* c, const char* key, int* out);
+/* getValueOfKey<std::complex<double>>: "(re,im)". */
+int rift_config_complex(const rift_config* c, const char* key, double* re, double* im);
+
+/* ---- Simulator driver (simulator.h:14-173, main.cc:40-67) -------------- */
+typedef struct rift_sim rift_sim;
+
+/* Simulator::OpenSimulation (simulator.cu:720-734): reads the keyword file
+ * (same keywords and defaults as ReadUserParams, simulator.cu:25-265), the
+ * rasters and series it names, builds the grid on the device, applies a
+ * read_hotstart file when named.  flow_rate is kept as the reference keeps it
+ * (stored, not used). */
+int rift_sim_open(const char* config_path, double flow_rate, rift_sim** out);
+/* Simulator::RunSimulation (simulator.cu:774-850): one time step plus the
+ * gridded series updates and the periodic output; *t receives the new time. */
+int rift_sim_step(rift_sim* s, double* t);
+/* The loop of main.cc:57-63: steps while t < t_end (t_end < 0: tf) and the
+ * run is not at steady state. */
+int rift_sim_run(rift_sim* s, double t_end);
+int rift_sim_time(rift_sim* s, double* t, double* dt, double* tf);
+/* The device grid of the run, for field access through rift_kp.h. */
+rift_grid* rift_sim_grid(rift_sim* s);
+/* Simulator::setSourceValue (simulator.cu:891-901): total source discharge,
+ * m^3/s, spread over the source cells. */
+int rift_sim_set_source_value(rift_sim* s, double discharge);
+/* Simulator::WriteHotstart / ReadHotstart (simulator.cu:912-1090), same
+ * binary layout. */
+int rift_sim_write_hotstart(rift_sim* s, const char* path);
+int rift_sim_read_hotstart(rift_sim* s, const char* path);
+/* Simulator::CloseSimulation (simulator.cu:735-772): summary maps, then the
+ * handle is freed. */
+int rift_sim_close(rift_sim* s);
+
 #ifdef __cplusplus
 }
 #endif

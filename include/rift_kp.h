@@ -187,6 +187,9 @@ int rift_set_tiles(rift_grid* g, const unsigned char* wet, const unsigned char* 
 int rift_get_tile_speeds(rift_grid* g, float* mx, float* my);
 /* Tile-grid geometry: nbx, nby (16 x 14 tiles, grid.cu:18-19). */
 int rift_get_tile_dims(rift_grid* g, int* nbx, int* nby);
+/* The descriptor the handle was created with (the values SetDeviceConstants
+ * stores in constant memory, grid.cu:321-345). */
+int rift_get_desc(rift_grid* g, rift_desc* out);
 /* Interpolated scalar rates used by the last step: hyetograph (m/s), drain
  * (m/s), hydrograph (per-cell m/s). */
 int rift_get_rates(rift_grid* g, float* out3);

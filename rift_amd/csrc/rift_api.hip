@@ -29,6 +29,14 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
+}  // namespace
+
+namespace rift {
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace rift
+
+namespace {
+
 #define HIPCHK(expr)                                                                   \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
@@ -809,6 +817,13 @@ int rift_get_tile_dims(rift_grid* g, int* nbx, int* nby)
     if (!g || !nbx || !nby) return fail(RIFT_E_ARG, "rift_get_tile_dims: null argument");
     *nbx = g->d.nbx;
     *nby = g->d.nby;
+    return RIFT_OK;
+}
+
+int rift_get_desc(rift_grid* g, rift_desc* out)
+{
+    if (!g || !out) return fail(RIFT_E_ARG, "rift_get_desc: null argument");
+    *out = g->desc;
     return RIFT_OK;
 }
 

@@ -82,6 +82,7 @@ SIGNATURES = [
     ("rift_set_tiles", C.c_int, [_VP, _U8, _U8]),
     ("rift_get_tile_speeds", C.c_int, [_VP, _F, _F]),
     ("rift_get_tile_dims", C.c_int, [_VP, _I, _I]),
+    ("rift_get_desc", C.c_int, [_VP, C.POINTER(RiftDesc)]),
     ("rift_get_rates", C.c_int, [_VP, _F]),
     ("rift_sum_interior", C.c_int, [_VP, C.c_int, _D]),
     ("rift_mass_balance", C.c_int, [_VP, _D]),
@@ -91,6 +92,26 @@ SIGNATURES = [
     ("rift_wet_cell_updates", C.c_int, [_VP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("rift_kappa", C.c_double, [C.c_double, C.c_double]),
     ("rift_geometry", C.c_int, [C.c_int, C.c_int, _D, _D, _D, C.c_double, _D, _D, _D, _D, _D, _D]),
+    ("rift_read_series", C.c_int, [C.c_char_p, _F, _F, C.c_int, _I]),
+    ("rift_read_raster", C.c_int, [C.c_char_p, C.c_int, _I, _I, _D, _D, C.c_long]),
+    ("rift_write_grid", C.c_int, [C.c_char_p, _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]),
+    ("rift_config_open", C.c_int, [C.c_char_p, C.POINTER(_VP)]),
+    ("rift_config_close", None, [_VP]),
+    ("rift_config_has", C.c_int, [_VP, C.c_char_p]),
+    ("rift_config_is_number", C.c_int, [_VP, C.c_char_p, _I]),
+    ("rift_config_string", C.c_int, [_VP, C.c_char_p, C.c_char_p, C.c_int, _I]),
+    ("rift_config_double", C.c_int, [_VP, C.c_char_p, _D]),
+    ("rift_config_bool", C.c_int, [_VP, C.c_char_p, _I]),
+    ("rift_config_complex", C.c_int, [_VP, C.c_char_p, _D, _D]),
+    ("rift_sim_open", C.c_int, [C.c_char_p, C.c_double, C.POINTER(_VP)]),
+    ("rift_sim_step", C.c_int, [_VP, _D]),
+    ("rift_sim_run", C.c_int, [_VP, C.c_double]),
+    ("rift_sim_time", C.c_int, [_VP, _D, _D, _D]),
+    ("rift_sim_grid", _VP, [_VP]),
+    ("rift_sim_set_source_value", C.c_int, [_VP, C.c_double]),
+    ("rift_sim_write_hotstart", C.c_int, [_VP, C.c_char_p]),
+    ("rift_sim_read_hotstart", C.c_int, [_VP, C.c_char_p]),
+    ("rift_sim_close", C.c_int, [_VP]),
 ]
 
 _lib = None
@@ -165,9 +186,27 @@ class Grid:
         self.nbx, self.nby = a.value, b.value
         self.ntiles = self.nbx * self.nby
 
+    @classmethod
+    def borrow(cls, handle):
+        """A Grid view of a handle owned elsewhere (a Simulation's grid);
+        closing the view does not destroy the handle."""
+        L = load()
+        self = cls.__new__(cls)
+        self.desc = RiftDesc()
+        _check(L.rift_get_desc(handle, C.byref(self.desc)), "rift_get_desc")
+        self.h = handle
+        self._owned = False
+        self.nx, self.ny, self.flags = self.desc.nx, self.desc.ny, self.desc.flags
+        a, b = C.c_int(), C.c_int()
+        _check(L.rift_get_tile_dims(self.h, C.byref(a), C.byref(b)), "rift_get_tile_dims")
+        self.nbx, self.nby = a.value, b.value
+        self.ntiles = self.nbx * self.nby
+        return self
+
     def close(self):
         if getattr(self, "h", None):
-            load().rift_destroy(self.h)
+            if getattr(self, "_owned", True):
+                load().rift_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -325,3 +364,140 @@ def grid_from_rasters(dem, dx, *, h0=None, n=None, n_const=0.0, K=None, psi=0.0,
     if per_phase:
         g.set_path(True)
     return g
+
+
+def read_series(path):
+    """CSV series (Source::ReadSource): (time_s, rate) float32 arrays."""
+    n = C.c_int()
+    _check(load().rift_read_series(os.fsencode(path), None, None, 0, C.byref(n)), "rift_read_series")
+    t = np.empty(n.value, np.float32)
+    r = np.empty(n.value, np.float32)
+    _check(load().rift_read_series(os.fsencode(path), t.ctypes.data_as(_F), r.ctypes.data_as(_F),
+                                   n.value, C.byref(n)), "rift_read_series")
+    return t, r
+
+
+def read_raster(path, allow_nodata=False):
+    """ESRI-ASCII raster: (data (nrows, ncols) row 0 = south, header dict)."""
+    nc, nr = C.c_int(), C.c_int()
+    hdr = np.empty(4)
+    _check(load().rift_read_raster(os.fsencode(path), int(allow_nodata), C.byref(nc), C.byref(nr),
+                                   _dp(hdr), None, 0), "rift_read_raster")
+    data = np.empty((nr.value, nc.value))
+    _check(load().rift_read_raster(os.fsencode(path), int(allow_nodata), C.byref(nc), C.byref(nr),
+                                   _dp(hdr), _dp(data), data.size), "rift_read_raster")
+    return data, {"xllcorner": hdr[0], "yllcorner": hdr[1], "cellsize": hdr[2], "nodata": hdr[3]}
+
+
+def write_grid(path, field, h_xll, h_yll, cellsize):
+    """writeGrid: the interior of an internal-grid field (ny, nx)."""
+    f = np.ascontiguousarray(field, dtype=np.float64)
+    ny, nx = f.shape
+    _check(load().rift_write_grid(os.fsencode(path), _dp(f), nx, ny, float(h_xll), float(h_yll),
+                                  float(cellsize)), "rift_write_grid")
+
+
+def write_raster(path, data, xll=0.0, yll=0.0, cellsize=1.0, nodata=-9999.0, fmt="%.10g"):
+    """Writes an ESRI-ASCII raster (row 0 of `data` = south) for inputs."""
+    data = np.asarray(data, dtype=np.float64)
+    nr, nc = data.shape
+    with open(path, "w") as f:
+        f.write(f"ncols {nc}\nnrows {nr}\nxllcorner {xll!r}\nyllcorner {yll!r}\n"
+                f"cellsize {cellsize!r}\nNODATA_value {nodata!r}\n")
+        for j in range(nr - 1, -1, -1):
+            f.write(" ".join(fmt % v for v in data[j]) + "\n")
+
+
+class Config:
+    """The keyword file (ConfigFile, config.h:48-209) through the C ABI."""
+
+    def __init__(self, path):
+        h = _VP()
+        _check(load().rift_config_open(os.fsencode(path), C.byref(h)), "rift_config_open")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().rift_config_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def has(self, key):
+        return bool(load().rift_config_has(self.h, key.encode()))
+
+    def is_number(self, key):
+        v = C.c_int()
+        _check(load().rift_config_is_number(self.h, key.encode(), C.byref(v)), "rift_config_is_number")
+        return bool(v.value)
+
+    def string(self, key):
+        n = C.c_int()
+        _check(load().rift_config_string(self.h, key.encode(), None, 0, C.byref(n)), "rift_config_string")
+        buf = C.create_string_buffer(n.value + 1)
+        _check(load().rift_config_string(self.h, key.encode(), buf, n.value + 1, None), "rift_config_string")
+        return buf.value.decode()
+
+    def double(self, key):
+        v = C.c_double()
+        _check(load().rift_config_double(self.h, key.encode(), C.byref(v)), "rift_config_double")
+        return v.value
+
+    def bool(self, key):
+        v = C.c_int()
+        _check(load().rift_config_bool(self.h, key.encode(), C.byref(v)), "rift_config_bool")
+        return bool(v.value)
+
+    def complex(self, key):
+        a, b = C.c_double(), C.c_double()
+        _check(load().rift_config_complex(self.h, key.encode(), C.byref(a), C.byref(b)), "rift_config_complex")
+        return complex(a.value, b.value)
+
+
+class Simulation:
+    """The Simulator driver (simulator.h:14-173) through the C ABI:
+    open = OpenSimulation, step = RunSimulation, close = CloseSimulation."""
+
+    def __init__(self, config_path, flow_rate=0.0):
+        h = _VP()
+        _check(load().rift_sim_open(os.fsencode(config_path), float(flow_rate), C.byref(h)), "rift_sim_open")
+        self.h = h
+        self.grid = Grid.borrow(load().rift_sim_grid(self.h))
+
+    def step(self):
+        t = C.c_double()
+        _check(load().rift_sim_step(self.h, C.byref(t)), "rift_sim_step")
+        return t.value
+
+    def run(self, t_end=-1.0):
+        _check(load().rift_sim_run(self.h, float(t_end)), "rift_sim_run")
+
+    def time(self):
+        t, dt, tf = C.c_double(), C.c_double(), C.c_double()
+        _check(load().rift_sim_time(self.h, C.byref(t), C.byref(dt), C.byref(tf)), "rift_sim_time")
+        return t.value, dt.value, tf.value
+
+    def set_source_value(self, q):
+        _check(load().rift_sim_set_source_value(self.h, float(q)), "rift_sim_set_source_value")
+
+    def write_hotstart(self, path):
+        _check(load().rift_sim_write_hotstart(self.h, os.fsencode(path)), "rift_sim_write_hotstart")
+
+    def read_hotstart(self, path):
+        _check(load().rift_sim_read_hotstart(self.h, os.fsencode(path)), "rift_sim_read_hotstart")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.grid.close()
+            h, self.h = self.h, None
+            _check(load().rift_sim_close(h), "rift_sim_close")
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

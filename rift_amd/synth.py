@@ -155,3 +155,70 @@ def series_hours_to_seconds(hours, values):
         t = np.append(t, np.float32(t[-1] + est))
         v = np.append(v, np.float32(0))
     return t.astype(np.float32), v.astype(np.float32)
+
+
+def _write_asc(path, a, cellsize, nodata=-9999.0):
+    """ESRI-ASCII raster of a corner array (row 0 = south), lower-left corner
+    at the origin; values in 17 significant digits so they read back exactly."""
+    a = np.asarray(a, dtype=np.float64)
+    nr, nc = a.shape
+    with open(path, "w") as f:
+        f.write(f"ncols {nc}\nnrows {nr}\nxllcorner 0\nyllcorner 0\n"
+                f"cellsize {cellsize!r}\nNODATA_value {nodata!r}\n")
+        for j in range(nr - 1, -1, -1):
+            f.write(" ".join("%.17g" % v for v in a[j]) + "\n")
+
+
+def _write_csv(path, hours, values, header):
+    with open(path, "w") as f:
+        f.write(f"# {header}\n")
+        for h, v in zip(hours, values):
+            f.write("%.9g,%.9g\n" % (float(h), float(v)))
+
+
+def write_case(case, outdir, name="case", **keywords):
+    """Writes a generated workload as the files a RIFT run reads: the DEM and
+    corner rasters (`.asc`), CSV series, surge maps `surge-###.txt`, and the
+    keyword file `<name>.cfg` (projected metres, lower-left corner at the
+    origin).  `keywords` add or override configuration keywords (e.g.
+    save_max=1, dt_print=60).  Returns the path of the keyword file."""
+    import os
+    os.makedirs(outdir, exist_ok=True)
+    dx = float(case["dx"])
+    cfg = {"DEM": "dem.asc", "projected": 1, "t0": 0, "tf": case["tf"], "output": "."}
+    _write_asc(os.path.join(outdir, "dem.asc"), case["dem"], dx)
+    if "n" in case:
+        _write_asc(os.path.join(outdir, "n.asc"), case["n"], dx)
+        cfg["n"] = "n.asc"
+    else:
+        cfg["n"] = case.get("n_const", 0.0)
+    if "h0" in case:
+        _write_asc(os.path.join(outdir, "h0.asc"), case["h0"], dx)
+        cfg["h0"] = "h0.asc"
+    if "K" in case:
+        _write_asc(os.path.join(outdir, "K.asc"), case["K"], dx)
+        cfg.update({"K": "K.asc", "psi": case["psi"], "dtheta": case["dtheta"]})
+    if "hyeto_hours" in case:
+        _write_csv(os.path.join(outdir, "hyetograph.csv"), case["hyeto_hours"], case["hyeto_mmhr"],
+                   "hours, mm/hr")
+        cfg["hyetograph"] = "hyetograph.csv"
+    if "hydro_hours" in case:
+        _write_csv(os.path.join(outdir, "hydrograph.csv"), case["hydro_hours"], case["hydro_cfs"],
+                   "hours, cfs")
+        i0, i1, j0, j1 = case["source_rect"]
+        # Coord2Index: i = (int)((X - h_xll) / cellsize + 0.5), h_xll = dx / 2
+        x = lambda i: 0.5 * dx + i * dx
+        cfg["hydrograph"] = "hydrograph.csv"
+        cfg["source_rect_X"] = f"({x(i0)!r},{x(i1)!r})"
+        cfg["source_rect_Y"] = f"({x(j0)!r},{x(j1)!r})"
+    if "surge_maps" in case:
+        for k, m in enumerate(case["surge_maps"]):
+            _write_asc(os.path.join(outdir, "surge-%03d.txt" % k), m, dx)
+        cfg.update({"surge_prefix": "surge", "surge_dt": case["surge_dt"], "surge_tf": case["surge_tf"]})
+    cfg.update(keywords)
+    path = os.path.join(outdir, f"{name}.cfg")
+    with open(path, "w") as f:
+        f.write(f"; generated workload {name}\n")
+        for k, v in cfg.items():
+            f.write(f"{k} = {v!r}\n" if isinstance(v, float) else f"{k} = {v}\n")
+    return path
