@@ -136,21 +136,23 @@ def test_simulator_gridded_rain_and_drain(rift, tmp_path, monkeypatch, interp):
     maps, with the four-corner nodata rule of p_interp (nnd > 2)."""
     n = 40
     N = n + 1
-    c = {"dem": 8.0 * synth.fractal_dem(n, 0.6, 13), "dx": 4.0, "n_const": 0.03, "tf": 40.0}
-    rain = np.empty((5, N, N))
-    for k in range(5):
+    c = {"dem": 8.0 * synth.fractal_dem(n, 0.6, 13), "dx": 4.0, "n_const": 0.03, "tf": 200.0}
+    nm = 7
+    rain = np.empty((nm, N, N))
+    for k in range(nm):
         rain[k] = 40.0 + 60.0 * synth.uniform(100 + k, N * N).reshape(N, N) + 15.0 * k
     rain[:, 10:14, 5:9] = 0.0
-    drain = np.full((5, N, N), 5.0)
+    drain = np.full((nm, N, N), 5.0)
     drain[:, 20:, :] = 25.0
-    c.update(rain_maps=rain, rain_dt=12, rain_tf=50.0, drain_maps=drain, drain_dt=12, drain_tf=50.0)
+    c.update(rain_maps=rain, rain_dt=50, rain_tf=230.0, drain_maps=drain, drain_dt=50,
+             drain_tf=230.0)
     case_dir = tmp_path / "rain"
     case_dir.mkdir()
-    for k in range(5):
+    for k in range(nm):
         synth._write_asc(str(case_dir / ("rain-%03d.txt" % k)), rain[k], c["dx"])
         synth._write_asc(str(case_dir / ("drain-%03d.txt" % k)), drain[k], c["dx"])
-    kw = dict(hyetograph_prefix="rain", hyetograph_dt=12, hyetograph_tf=50.0,
-              drain_prefix="drain", drain_dt=12, drain_tf=50.0)
+    kw = dict(hyetograph_prefix="rain", hyetograph_dt=50, hyetograph_tf=230.0,
+              drain_prefix="drain", drain_dt=50, drain_tf=230.0)
     if interp:
         kw["hyetograph_interp"] = 1
     cfg = synth.write_case(c, str(case_dir), **kw)
@@ -160,11 +162,19 @@ def test_simulator_gridded_rain_and_drain(rift, tmp_path, monkeypatch, interp):
     print("simulator gridded rain interp=%d" % interp, flush=True)
     sim, _, steps = _lockstep(rift, cfg, o)
     g = sim.grid
-    assert steps > 20
+    assert steps >= 10
+    # held maps are equal up to float storage; interpolated ones also move
+    # with the small difference of the two clocks (the time factor)
+    t_gpu, _, _ = sim.time()
+    t_orc, _ = o.time
+    slope = np.abs(np.diff(rain, axis=0)).max() / 3.6e6 / 50.0
     for f in ("rainmap", "drainmap"):
         a = g.interior(g.field(f))
         b = o.interior(o.field(f))
-        assert np.abs(a - b).max() <= 1e-6 * max(np.abs(b).max(), 1e-12) + 1e-15, f
+        tol = 1e-6 * max(np.abs(b).max(), 1e-12) + 1e-15
+        if interp and f == "rainmap":
+            tol += slope * abs(t_gpu - t_orc)
+        assert np.abs(a - b).max() <= tol, f
     dh = np.abs(g.interior(g.field("depth")) - o.interior(o.depth())).max()
     assert dh <= DH_TOL, dh
     sim.close()
