@@ -128,6 +128,9 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-size reference "
                          "for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="launch variant of the marching kernels (default: autotuned after the "
+                         "warm-up, outside the timed region)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -143,6 +146,14 @@ def main():
     _, wcu0 = g.wet_cells()
     wet, act, n_act = g.tiles()
     print(f"[bench] warm-up done: {n_act}/{g.ntiles} tiles active", file=sys.stderr, flush=True)
+    variant_ms = None
+    if args.variant >= 0:
+        g.set_variant(args.variant)
+        variant = args.variant
+    else:
+        variant, variant_ms = g.autotune(16)
+        print(f"[bench] autotuned launch variant {variant}: "
+              + " ".join(f"{m:.4f}" for m in variant_ms) + " ms/step", file=sys.stderr, flush=True)
 
     # timed region: K steps enqueued back to back, stream synchronised on both sides
     g.sync()
@@ -209,6 +220,8 @@ def main():
                           "frac": value * STEP_BYTES_PER_WCU / HBM_PEAK},
         "kernel_ms_per_step": {"march1": ms1 / kp_steps, "march2": ms2 / kp_steps,
                                "grow": msg / kp_steps, "timestep": mst / kp_steps},
+        "launch_variant": variant,
+        "variant_ms_per_step": None if variant_ms is None else [float(m) for m in variant_ms],
         "sim_time_s": t_sim,
         "mass_balance_m3": {"stored": mb[0], "infiltrated": mb[1], "rain_in": mb[2]},
     }

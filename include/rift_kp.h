@@ -164,7 +164,23 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
 /* RIFT_OPT_GRAPHS: 1 (default) = replay the fused step from captured HIP
  * graphs (16 steps per launch), 0 = launch its kernels one by one. */
 #define RIFT_OPT_GRAPHS 2
+/* RIFT_OPT_VARIANT: launch variant of the fused step's marching kernels,
+ * 0 .. RIFT_NVARIANTS-1 (default 0): bit 0 = stage 1 with a larger register
+ * budget (4 instead of 5 waves per SIMD), bit 1 = stage 2 likewise (3
+ * instead of 4), bit 2 = plain grid-stride work order instead of the
+ * XCD-banded one.  Every variant computes the same results bit for bit. */
+#define RIFT_OPT_VARIANT 3
+#define RIFT_NVARIANTS 8
 int rift_set_option(rift_grid* g, int option, int value);
+
+/* Autotuner (no counterpart in the reference, whose 16 x 14 launch shape is
+ * fixed, grid.cu:18-19): times every launch variant of the fused step on a
+ * private copy of the handle's current state (nsteps graph-replayed steps
+ * each, after two warm-up steps), selects the fastest on the handle and
+ * returns it in *best; ms_per_step (optional, RIFT_NVARIANTS entries)
+ * receives each variant's time.  The handle's own state is not advanced.
+ * Needs device memory for a second copy of the domain. */
+int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step);
 
 /* Wait for all queued work of the handle. */
 int rift_sync(rift_grid* g);

@@ -47,11 +47,14 @@ __device__ __forceinline__ float minmod3(float a, float b, float c)
     }
 }
 
-// limiter(U, Up1, Um1) of grid.cu:78-80 from dr = Up1-U, dl = U-Um1.
+// limiter(U, Up1, Um1) of grid.cu:78-80 from dr = Up1-U, dl = U-Um1, halved:
+// 0.5 minmod(theta dr, (dr+dl)/2, theta dl).  Scaling by a power of two
+// commutes with rounding, so the halving is folded into the operands
+// (0.65f = 1.3f/2 exactly).
 template <bool Textbook>
 __device__ __forceinline__ float half_slope(float dr, float dl)
 {
-    return 0.5f * minmod3<Textbook>(kTheta * dr, 0.5f * (dr + dl), kTheta * dl);
+    return minmod3<Textbook>((0.5f * kTheta) * dr, 0.25f * (dr + dl), (0.5f * kTheta) * dl);
 }
 
 // Both face states of one cell (Construct, grid.cu:91-116).
@@ -73,12 +76,16 @@ __device__ __forceinline__ FaceStates construct(float hm1, float h, float hp1,
     float dr = (hp1 - h) + (bp1 - b);
     float dl = (h - hm1) + (b - bm1);
     float sw = half_slope<Textbook>(dr, dl);
-    float hp = (h + (b - Bp)) + sw;
-    float hmn = (h + (b - Bm)) - sw;
-    // positivity fix: both branches use the pre-fix sum (wBar, grid.cu:108-115)
-    float sum = hp + hmn;
-    if (hp < 0.f) { hp = 0.f; hmn = sum; }
-    if (hmn < 0.f) { hp = sum; hmn = 0.f; }
+    const float hp0 = (h + (b - Bp)) + sw;
+    const float hm0 = (h + (b - Bm)) - sw;
+    // positivity fix, both branches with the pre-fix sum s (wBar,
+    // grid.cu:108-115):  hp < 0 -> (0, s);  then hm < 0 -> (s, 0).  As pure
+    // selections: hp' = min(med3(hp, 0, s), s), hm' = max(med3(hm, 0, s), 0)
+    // (for s >= 0 the median clips each side into [0, s]; for s < 0 the
+    // result is (s, 0) whatever the signs were).
+    const float sum = hp0 + hm0;
+    const float hp = fminf(__builtin_amdgcn_fmed3f(hp0, 0.f, sum), sum);
+    const float hmn = fmaxf(__builtin_amdgcn_fmed3f(hm0, 0.f, sum), 0.f);
     float sx = half_slope<Textbook>(qxp1 - qx, qx - qxm1);
     float sy = half_slope<Textbook>(qyp1 - qy, qy - qym1);
     s.hp = hp;
@@ -92,10 +99,13 @@ __device__ __forceinline__ FaceStates construct(float hm1, float h, float hp1,
 
 // Desingularised velocity numerator factor: u = 2 h q / (h^2 + max(h^2, kappa))
 // (getVelocity, grid.cu:82-84); returns 2h/(h^2+max(h^2,kappa)).
+__device__ __forceinline__ float vel_factor_h2(float h, float h2, float kappa)
+{
+    return (2.f * h) * frcp(h2 + fmaxf(h2, kappa));
+}
 __device__ __forceinline__ float vel_factor(float h, float kappa)
 {
-    float h2 = h * h;
-    return 2.f * h * frcp(h2 + fmaxf(h2, kappa));
+    return vel_factor_h2(h, h * h, kappa);
 }
 
 struct Flux {
@@ -109,46 +119,53 @@ template <int Dir>
 __device__ __forceinline__ Flux face_flux(float hL, float qxL, float qyL,
                                           float hR, float qxR, float qyR, float kappa)
 {
-    float fL = vel_factor(hL, kappa), fR = vel_factor(hR, kappa);
-    float uL = fL * qxL, vL = fL * qyL, uR = fR * qxR, vR = fR * qyR;
+    const float h2L = hL * hL, h2R = hR * hR;
+    const float fL = vel_factor_h2(hL, h2L, kappa), fR = vel_factor_h2(hR, h2R, kappa);
+    const float uL = fL * qxL, vL = fL * qyL, uR = fR * qxR, vR = fR * qyR;
     // momenta recomputed from the desingularised velocities (grid.cu:133-136)
-    float mxL = hL * uL, myL = hL * vL, mxR = hR * uR, myR = hR * vR;
-    float cL = (Dir == 0) ? uL : vL;
-    float cR = (Dir == 0) ? uR : vR;
-    float sL = fsqrt(fmaxf(kG * hL, 0.f)), sR = fsqrt(fmaxf(kG * hR, 0.f));
-    // sqrtf of a negative depth is NaN in the reference and fmaxf/fminf drop
-    // it: a side with h < 0 does not bound the speeds.
-    bool okL = hL >= 0.f, okR = hR >= 0.f;
-    float ap = fmaxf(fmaxf(okL ? cL + sL : -INFINITY, okR ? cR + sR : -INFINITY), 0.f);
-    float am = fminf(fminf(okL ? cL - sL : INFINITY, okR ? cR - sR : INFINITY), 0.f);
+    const float mxL = hL * uL, myL = hL * vL, mxR = hR * uR, myR = hR * vR;
+    const float cL = (Dir == 0) ? uL : vL;
+    const float cR = (Dir == 0) ? uR : vR;
+    // sqrtf of a negative face depth is NaN, as in the reference, and the
+    // IEEE maxNum/minNum of v_max_f32/v_min_f32 drop it exactly as fmaxf and
+    // fminf do there: a side with h < 0 does not bound the speeds.
+    const float sL = fsqrt(kG * hL), sR = fsqrt(kG * hR);
+    const float ap = fmaxf(fmaxf(cL + sL, cR + sR), 0.f);
+    const float am = fminf(fminf(cL - sL, cR - sR), 0.f);
 
-    // physical fluxes, zero on a dry side (hu^2/h = h u^2, hu hv / h = h u v)
+    // physical fluxes (hu^2/h = h u^2, hu hv / h = h u v)
     const float hg = 0.5f * kG;
     float F0L, F1L, F2L, F0R, F1R, F2R;
     if (Dir == 0) {
-        F0L = mxL; F1L = mxL * uL + hg * hL * hL; F2L = mxL * vL;
-        F0R = mxR; F1R = mxR * uR + hg * hR * hR; F2R = mxR * vR;
+        F0L = mxL; F1L = fmaf(mxL, uL, hg * h2L); F2L = mxL * vL;
+        F0R = mxR; F1R = fmaf(mxR, uR, hg * h2R); F2R = mxR * vR;
     } else {
-        F0L = myL; F1L = mxL * vL; F2L = myL * vL + hg * hL * hL;
-        F0R = myR; F1R = mxR * vR; F2R = myR * vR + hg * hR * hR;
+        F0L = myL; F1L = mxL * vL; F2L = fmaf(myL, vL, hg * h2L);
+        F0R = myR; F1R = mxR * vR; F2R = fmaf(myR, vR, hg * h2R);
     }
-    if (!(hL > 0.f)) { F0L = 0.f; F1L = 0.f; F2L = 0.f; }
-    if (!(hR > 0.f)) { F0R = 0.f; F1R = 0.f; F2R = 0.f; }
 
+    // computeH (grid.cu:86-89): (ap FL - am FR)/(ap-am) + ap am/(ap-am) (UR-UL),
+    // zero when ap - am = 0 (then ap = am = 0 and every term vanishes).  A dry
+    // side's physical flux is zero: its weight is zeroed instead of its three
+    // flux components (the fluxes are finite, so the products are the same).
+    const float den = ap - am;
+    const float r = (den > 0.f) ? frcp(den) : 0.f;
+    const float a = ap * r, b = am * r, c = a * am;
+    const float aL = (hL > 0.f) ? a : 0.f;
+    const float bR = (hR > 0.f) ? b : 0.f;
     Flux out;
-    float den = ap - am;
-    if (den > 0.f) {
-        float r = frcp(den);
-        float a = ap * r, b = am * r, c = ap * am * r;
-        // computeH (grid.cu:86-89): (ap FL - am FR)/(ap-am) + ap am/(ap-am) (UR-UL)
-        out.f0 = (a * F0L - b * F0R) + c * (hR - hL);
-        out.f1 = (a * F1L - b * F1R) + c * (mxR - mxL);
-        out.f2 = (a * F2L - b * F2R) + c * (myR - myL);
-    } else {
-        out.f0 = 0.f; out.f1 = 0.f; out.f2 = 0.f;
-    }
+    out.f0 = fmaf(-bR, F0R, fmaf(aL, F0L, c * (hR - hL)));
+    out.f1 = fmaf(-bR, F1R, fmaf(aL, F1L, c * (mxR - mxL)));
+    out.f2 = fmaf(-bR, F2R, fmaf(aL, F2L, c * (myR - myL)));
     out.speed = fmaxf(ap, -am);
     return out;
+}
+
+// Flux part of dU for one cell: -(E - W)/dx - (N - S)/dy.
+__device__ __forceinline__ float flux_divergence(float e, float w, float n, float s, float inv_dx,
+                                                 float inv_dy)
+{
+    return fmaf(s - n, inv_dy, (w - e) * inv_dx);
 }
 
 // Manning coefficient G (grid.cu:753-758):

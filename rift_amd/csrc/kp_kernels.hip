@@ -21,26 +21,6 @@
 
 namespace rift {
 
-// ---------------------------------------------------------------------------
-// Scalar forcing: Source::InterpolateRate (source.cc:65-73) in float with
-// correctly rounded operations, then UpdateSource's unit division
-// (simulator.cu:500-538).  The previous value persists when t is outside the
-// table, and is divided again, as in the reference.
-// ---------------------------------------------------------------------------
-__device__ static float series_interp(const SeriesTable& s, float tf, float rate)
-{
-    for (int i = 0; i < s.n - 1; ++i) {
-        if (tf >= s.time[i] && tf < s.time[i + 1]) {
-            float num = __fsub_rn(s.rate[i + 1], s.rate[i]);
-            float den = __fsub_rn(s.time[i + 1], s.time[i]);
-            float q = __fdiv_rn(num, den);
-            float dd = __fsub_rn(tf, s.time[i]);
-            rate = __fadd_rn(s.rate[i], __fmul_rn(q, dd));
-        }
-    }
-    return rate;
-}
-
 __global__ void k_reset_counters(Dev d)
 {
     if (threadIdx.x != 0) return;
@@ -55,32 +35,15 @@ __global__ void k_begin_step(Dev d)
     Scalars* sc = d.sc;
     sc->n_active = 0;
     sc->n_units = 0;
+    sc->n_units_acc = 0;
     sc->wet_cells = 0ull;
-    const unsigned f = d.flags;
-    if (!(f & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG))) return;
-    const float tf = (float)sc->t;
-    if (f & RIFT_HYDROGRAPH) {
-        double value;
-        if (sc->hydro_external) {
-            value = sc->hydro_value;
-        } else {
-            float r = series_interp(d.ser[RIFT_SER_HYDROGRAPH], tf, sc->hydro_rate);
-            r = __fdiv_rn(r, 35.3147f);               // cfs -> m^3/s
-            sc->hydro_rate = r;
-            value = (double)r;
-        }
-        // setSourceValue, simulator.cu:891-901
-        double per_cell = value / d.dx / d.dy / (double)(d.nsrc > 0 ? d.nsrc : 1);
-        sc->src_cell_rate = (float)per_cell;
-    }
-    if (f & RIFT_RAIN_AVG) {
-        float r = series_interp(d.ser[RIFT_SER_HYETOGRAPH], tf, sc->rain_rate);
-        sc->rain_rate = __fdiv_rn(r, 3600.f * 1000.f);  // mm/hr -> m/s
-    }
-    if (f & RIFT_DRAIN_AVG) {
-        float r = series_interp(d.ser[RIFT_SER_DRAIN], tf, sc->drain_rate);
-        sc->drain_rate = __fdiv_rn(r, 3600.f * 1000.f);
-    }
+    sc->next_valid = 0;
+    if (!(d.flags & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG))) return;
+    const Forcing f = forcing_at(d, sc);
+    sc->rain_rate = f.rain;
+    sc->drain_rate = f.drain;
+    sc->hydro_rate = f.hydro;
+    sc->src_cell_rate = f.cell;
 }
 
 // ---------------------------------------------------------------------------
@@ -252,12 +215,12 @@ __global__ __launch_bounds__(256) void k_flux(Dev d)
                 const int yn = (ty + 1) * kTileW + tx, ys = ty * kTileW + tx;
                 smx = sSx[xe];
                 smy = sSy[yn];
-                float dh = -(sFx[xe * 3 + 0] - sFx[xw * 3 + 0]) * d.inv_dx;
-                float dhu = -(sFx[xe * 3 + 1] - sFx[xw * 3 + 1]) * d.inv_dx;
-                float dhv = -(sFx[xe * 3 + 2] - sFx[xw * 3 + 2]) * d.inv_dx;
-                dh += -(sFy[yn * 3 + 0] - sFy[ys * 3 + 0]) * d.inv_dy;
-                dhu += -(sFy[yn * 3 + 1] - sFy[ys * 3 + 1]) * d.inv_dy;
-                dhv += -(sFy[yn * 3 + 2] - sFy[ys * 3 + 2]) * d.inv_dy;
+                const float dh = flux_divergence(sFx[xe * 3 + 0], sFx[xw * 3 + 0], sFy[yn * 3 + 0],
+                                                 sFy[ys * 3 + 0], d.inv_dx, d.inv_dy);
+                const float dhu = flux_divergence(sFx[xe * 3 + 1], sFx[xw * 3 + 1], sFy[yn * 3 + 1],
+                                                  sFy[ys * 3 + 1], d.inv_dx, d.inv_dy);
+                const float dhv = flux_divergence(sFx[xe * 3 + 2], sFx[xw * 3 + 2], sFy[yn * 3 + 2],
+                                                  sFy[ys * 3 + 2], d.inv_dx, d.inv_dy);
 
                 const int e = (ty + 2) * WX + (tx + 2);
                 const size_t q = cell(d, i, j);
@@ -425,21 +388,21 @@ __global__ __launch_bounds__(256) void k_integrate1(Dev d)
                 if (d.flags & RIFT_SURGE_GRID) {
                     float sg = d.map[RIFT_MAP_SURGE][0][q];
                     if (sg != d.nodata && sg > 0.f) {
-                        float next_h = h + dh * dt;
+                        float next_h = fmaf(dh, dt, h);
                         dh = (sg - next_h) * inv_dt;
                         d.dh[q] = dh;
                     }
                 }
-                float hn = h + dh * dt;
-                const float rden = frcp(1.f - dt * d.G[q]);
-                hu = (hu + d.dhu[q] * dt) * rden;
-                hv = (hv + d.dhv[q] * dt) * rden;
+                float hn = fmaf(dh, dt, h);
+                const float rden = frcp(fmaf(-dt, d.G[q], 1.f));
+                hu = fmaf(d.dhu[q], dt, hu) * rden;
+                hv = fmaf(d.dhv[q], dt, hv) * rden;
                 float hc = (hn > kEps) ? hn : 0.f;
                 d.h[q] = hc; d.hu[q] = hu; d.hv[q] = hv;
                 if (d.flags & RIFT_INFILTRATION) {
                     float F = d.F[q];
                     d.F0[q] = F;
-                    d.F[q] = F + d.dF[q] * dt;
+                    d.F[q] = fmaf(d.dF[q], dt, F);
                 }
                 if (euler) {
                     diagnostics(d, q, hc, hu, hv, tnow, true);
@@ -473,13 +436,13 @@ __global__ __launch_bounds__(256) void k_integrate2(Dev d)
                 const size_t q = cell(d, i, j);
                 float hs = d.h[q];
                 hs = (hs > kEps) ? hs : 0.f;
-                float hn = 0.5f * (d.h0[q] + hs + d.dh[q] * dt);
-                float hu = 0.5f * (d.hu0[q] + d.hu[q] + d.dhu[q] * dt);
-                float hv = 0.5f * (d.hv0[q] + d.hv[q] + d.dhv[q] * dt);
+                float hn = 0.5f * fmaf(d.dh[q], dt, d.h0[q] + hs);
+                float hu = 0.5f * fmaf(d.dhu[q], dt, d.hu0[q] + d.hu[q]);
+                float hv = 0.5f * fmaf(d.dhv[q], dt, d.hv0[q] + d.hv[q]);
                 float hc = (hn > kEps) ? hn : 0.f;
                 d.h[q] = hc; d.hu[q] = hu; d.hv[q] = hv;
                 if (d.flags & RIFT_INFILTRATION)
-                    d.F[q] = 0.5f * (d.F0[q] + d.F[q] + d.dF[q] * dt);
+                    d.F[q] = 0.5f * fmaf(d.dF[q], dt, d.F0[q] + d.F[q]);
                 diagnostics(d, q, hc, hu, hv, tnow, false);
                 any = cell_wet(d, i, j, q, hc, false);
             }

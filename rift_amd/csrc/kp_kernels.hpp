@@ -43,7 +43,15 @@ struct Scalars {
     unsigned long long wcu_total;    // cumulative wet-cell updates since init
     // FP64 mass accounting (m^3 / (dx dy) until scaled on the host)
     double v_rain, v_src, v_drain, v_out;
-    int pad;
+    // fused step: units compacted by grow2 (read by stage 1; moved to
+    // n_units by the dt kernel, which zeroes it for the next step), and the
+    // forcing of the next step, evaluated by the dt kernel at the new time
+    // and installed by the next grow2 when next_valid is set
+    int n_units_acc;
+    int next_valid;
+    int n_active_acc;                 // dt kernel: cross-workgroup accumulators
+    unsigned long long wet_cells_acc;
+    float nxt_rain, nxt_drain, nxt_hydro, nxt_cell;
 };
 
 struct Dev {
@@ -63,6 +71,7 @@ struct Dev {
     long long soff;              // h0 - h (elements): offset of the second buffer
     int n_uniform;
     float n_const;
+    int band;                    // XCD-banded order of dense unit lists
     float *mx, *my;              // per-tile maxima of the last flux pass
     float *mx1, *my1;            // fused step: stage-1 maxima of this step
     Scalars* sc;
@@ -78,6 +87,58 @@ struct Dev {
 __host__ __device__ inline size_t cell(const Dev& d, int i, int j)
 {
     return (size_t)j * (size_t)d.P + (size_t)i;
+}
+
+// Source::InterpolateRate (source.cc:65-73) in float: inside [t_i, t_i+1) the
+// linear interpolant; outside the table the previous rate is kept (and the
+// caller divides it by the unit again, as the reference does).
+__device__ inline float series_interp(const SeriesTable& s, float tf, float rate)
+{
+    for (int i = 0; i < s.n - 1; ++i) {
+        if (tf >= s.time[i] && tf < s.time[i + 1]) {
+            float num = __fsub_rn(s.rate[i + 1], s.rate[i]);
+            float den = __fsub_rn(s.time[i + 1], s.time[i]);
+            float q = __fdiv_rn(num, den);
+            float dd = __fsub_rn(tf, s.time[i]);
+            rate = __fadd_rn(s.rate[i], __fmul_rn(q, dd));
+        }
+    }
+    return rate;
+}
+
+struct Forcing {
+    float rain, drain, hydro, cell;
+};
+
+// UpdateSource + setSourceValue (simulator.cu:500-538, 891-901) at the time
+// held in sc->t, from the rates of the previous step.
+__device__ inline Forcing forcing_at(const Dev& d, const Scalars* sc)
+{
+    Forcing r{sc->rain_rate, sc->drain_rate, sc->hydro_rate, sc->src_cell_rate};
+    const unsigned f = d.flags;
+    const float tf = (float)sc->t;
+    if (f & RIFT_HYDROGRAPH) {
+        double value;
+        if (sc->hydro_external) {
+            value = sc->hydro_value;
+        } else {
+            float q = series_interp(d.ser[RIFT_SER_HYDROGRAPH], tf, r.hydro);
+            q = __fdiv_rn(q, 35.3147f);               // cfs -> m^3/s
+            r.hydro = q;
+            value = (double)q;
+        }
+        const double per_cell = value / d.dx / d.dy / (double)(d.nsrc > 0 ? d.nsrc : 1);
+        r.cell = (float)per_cell;
+    }
+    if (f & RIFT_RAIN_AVG) {
+        const float q = series_interp(d.ser[RIFT_SER_HYETOGRAPH], tf, r.rain);
+        r.rain = __fdiv_rn(q, 3600.f * 1000.f);    // mm/hr -> m/s
+    }
+    if (f & RIFT_DRAIN_AVG) {
+        const float q = series_interp(d.ser[RIFT_SER_DRAIN], tf, r.drain);
+        r.drain = __fdiv_rn(q, 3600.f * 1000.f);
+    }
+    return r;
 }
 
 // ComputeTimestep's arithmetic (simulator.cu:549-561) on one lane; also the
@@ -116,7 +177,8 @@ void launch_sum_interior(const Dev& d, const float* x, int mode, double* out, hi
 constexpr int kStrip = 60;
 void launch_grow2(const Dev& d, hipStream_t s);
 void launch_timestep2(const Dev& d, hipStream_t s);
-void launch_march(const Dev& d, int stage, hipStream_t s);
+void launch_march(const Dev& d, int stage, hipStream_t s, int variant);
+constexpr int kMarchVariants = 4;
 void launch_canonicalize(const Dev& d, hipStream_t s);
 
 }  // namespace rift

@@ -25,6 +25,8 @@
 // grow2.  Inactive tiles are never touched, so no copies are needed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kp_device.hpp"
 #include "kp_kernels.hpp"
 
@@ -67,30 +69,39 @@ __device__ __forceinline__ void strip_tiles(const Dev& d, int s, int& txa, int& 
 
 // ---------------------------------------------------------------------------
 // grow2: threads [0, ntiles) handle tiles, [ntiles, ntiles+nunits) units.
+// Only the units are compacted (one atomic per wavefront): the marching
+// kernels walk units, and the dt kernel counts the active tiles and cells.
+// Thread 0 installs the forcing the last dt kernel evaluated for this step.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_grow2(Dev d)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        Scalars* sc = d.sc;
+        if (sc->next_valid) {
+            sc->rain_rate = sc->nxt_rain;
+            sc->drain_rate = sc->nxt_drain;
+            sc->hydro_rate = sc->nxt_hydro;
+            sc->src_cell_rate = sc->nxt_cell;
+            sc->next_valid = 0;
+        }
+    }
     const bool is_tile = t < d.ntiles;
     const bool is_unit = t >= d.ntiles && t < d.ntiles + d.nunits;
-    bool a = false, ua = false;
-    unsigned long long cells = 0ull;
+    bool ua = false;
     if (is_tile) {
         if (d.flip[t]) {   // stage 2 of the last step wrote the other buffer
             d.sel[t] ^= 1;
             d.flip[t] = 0;
         }
-        a = dilated(d, t);
+        const bool a = dilated(d, t);
         d.act[t] = a ? 1 : 0;
         if (a) {
             const int bx = t % d.nbx, by = t / d.nbx;
             const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
             const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
             unsigned units = 0;
-            if (i1 > i0 && j1 > j0) {
-                cells = (unsigned long long)((i1 - i0) * (j1 - j0));
-                units = (unsigned)((i1 - 1 - 2) / kStrip - (i0 - 2) / kStrip + 1);
-            }
+            if (i1 > i0 && j1 > j0) units = (unsigned)((i1 - 1 - 2) / kStrip - (i0 - 2) / kStrip + 1);
             d.mx[t] = 0.f;
             d.my[t] = 0.f;
             d.mx1[t] = 0.f;
@@ -106,23 +117,14 @@ __global__ __launch_bounds__(256) void k_grow2(Dev d)
         strip_tiles(d, s, txa, txb);
         for (int tx = txa; tx <= txb; ++tx) ua |= dilated(d, r * d.nbx + tx);
     }
+    // waves that hold tiles only skip the ballot (uniform per wavefront)
+    if (__ballot(is_unit) == 0ull) return;
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    unsigned long long m = __ballot(a), mu = __ballot(ua);
-    for (int off = 32; off > 0; off >>= 1) cells += __shfl_xor(cells, off);
-    int base = 0, ubase = 0;
-    if (lane == 0) {
-        int cnt = __popcll(m), ucnt = __popcll(mu);
-        if (cnt) {
-            base = atomicAdd(&d.sc->n_active, cnt);
-            atomicAdd(&d.sc->wet_cells, cells);
-            atomicAdd(&d.sc->wcu_total, cells);
-        }
-        if (ucnt) ubase = atomicAdd(&d.sc->n_units, ucnt);
-    }
-    base = __shfl(base, 0);
+    const unsigned long long mu = __ballot(ua);
+    int ubase = 0;
+    if (lane == 0 && mu) ubase = atomicAdd(&d.sc->n_units_acc, __popcll(mu));
     ubase = __shfl(ubase, 0);
-    if (a) d.list[base + __popcll(m & lt)] = t;
     if (ua) d.ulist[ubase + __popcll(mu & lt)] = t - d.ntiles;
 }
 
@@ -137,7 +139,7 @@ struct Raw {                 // global loads of one row, issued a row ahead
 struct Row {
     float h, hu, hv, b;     // state used by the flux (U^n in stage 1, U* in 2)
     float hraw;             // stage 2: U^n depth as stored (stage-1 sources)
-    float hn, hun, hvn;     // stage 2: clamped U^n of the cell (own-cell Heun)
+    float hun, hvn;         // stage 2: U^n momenta of the cell (own-cell Heun)
 };
 
 struct Src {                 // own-cell inputs of an output row
@@ -146,12 +148,27 @@ struct Src {                 // own-cell inputs of an output row
 
 // Per-lane constants of one unit.  The rows of a unit touch tile rows r-1,
 // r, r+1 only; their buffer selects and activity flags are read once.
+// Cells are addressed as a row pointer (wave-uniform, scalar registers) plus
+// a 32-bit byte offset per lane (the column, and one plane more for a tile
+// whose state lives in the second buffer), which the global loads and stores
+// take directly (scalar base + vector offset).
 struct Lane {
-    int c, sc, txo, r;
+    unsigned c4, sc4;        // byte offsets of the output / clamped column
+    unsigned pl4;            // bytes of one plane (second state buffer)
+    int txo, r;
     bool gx;
     bool sel0, sel1, sel2;   // buffer selects of tile rows r-1, r, r+1
     bool act0, act1, act2;   // activity of tile rows r-1, r, r+1
 };
+
+__device__ __forceinline__ float ld(const float* row, unsigned byte_off)
+{
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(row) + byte_off);
+}
+__device__ __forceinline__ float& at(float* row, unsigned byte_off)
+{
+    return *reinterpret_cast<float*>(reinterpret_cast<char*>(row) + byte_off);
+}
 
 __device__ __forceinline__ int tile_row_slot(const Lane& L, int sr)
 {
@@ -159,9 +176,15 @@ __device__ __forceinline__ int tile_row_slot(const Lane& L, int sr)
     return tr - L.r + 1;     // 0, 1 or 2 (uniform across the wave)
 }
 
-__device__ __forceinline__ bool pick(int slot, bool a, bool b, bool c)
+template <class T>
+__device__ __forceinline__ T pick(int slot, T a, T b, T c)
 {
     return slot == 0 ? a : (slot == 1 ? b : c);
+}
+
+__device__ __forceinline__ size_t row_of(const Dev& d, int j)
+{
+    return (size_t)j * (size_t)d.P;
 }
 
 template <int Stage>
@@ -169,33 +192,33 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const Lane& L, int j)
 {
     const int sr = min(max(j, 2), d.ny - 3);
     const int slot = tile_row_slot(L, sr);
-    const size_t q = cell(d, L.sc, sr);
-    const size_t qs = q + (pick(slot, L.sel0, L.sel1, L.sel2) ? (size_t)d.soff : 0);
+    const size_t rb = row_of(d, sr);
+    const unsigned lo = L.sc4 + (pick(slot, L.sel0, L.sel1, L.sel2) ? L.pl4 : 0u);
     Raw r;
-    r.h = d.h[qs];
-    r.hu = d.hu[qs];
-    r.hv = d.hv[qs];
-    r.b = d.BC[q];
-    r.by = d.BY[cell(d, L.c, j)];
+    r.h = ld(d.h + rb, lo);
+    r.hu = ld(d.hu + rb, lo);
+    r.hv = ld(d.hv + rb, lo);
+    r.b = ld(d.BC + rb, L.sc4);
+    r.by = ld(d.BY + row_of(d, j), L.c4);
     if constexpr (Stage == 2) {
-        r.dh = d.dh[q];
-        r.dhu = d.dhu[q];
-        r.dhv = d.dhv[q];
-        r.sg = (d.flags & RIFT_SURGE_GRID) ? d.map[RIFT_MAP_SURGE][0][q] : 0.f;
-        r.n = d.n_uniform ? d.n_const : d.n[q];
+        r.dh = ld(d.dh + rb, L.sc4);
+        r.dhu = ld(d.dhu + rb, L.sc4);
+        r.dhv = ld(d.dhv + rb, L.sc4);
+        r.sg = (d.flags & RIFT_SURGE_GRID) ? ld(d.map[RIFT_MAP_SURGE][0] + rb, L.sc4) : 0.f;
+        r.n = d.n_uniform ? d.n_const : ld(d.n + rb, L.sc4);
     }
     return r;
 }
 
 __device__ __forceinline__ Src fetch_src(const Dev& d, const Lane& L, int j)
 {
-    const size_t q = cell(d, L.c, j);
+    const size_t rb = row_of(d, j);
     Src s;
-    s.bx = d.BX[q];
-    s.K = (d.flags & RIFT_INFILTRATION) ? d.K[q] : 0.f;
-    s.F = (d.flags & RIFT_INFILTRATION) ? d.F[q] : 0.f;
-    s.rain = (d.flags & RIFT_RAIN_GRID) ? d.map[RIFT_MAP_RAIN][0][q] : 0.f;
-    s.drain = (d.flags & RIFT_DRAIN_GRID) ? d.map[RIFT_MAP_DRAIN][0][q] : 0.f;
+    s.bx = ld(d.BX + rb, L.c4);
+    s.K = (d.flags & RIFT_INFILTRATION) ? ld(d.K + rb, L.c4) : 0.f;
+    s.F = (d.flags & RIFT_INFILTRATION) ? ld(d.F + rb, L.c4) : 0.f;
+    s.rain = (d.flags & RIFT_RAIN_GRID) ? ld(d.map[RIFT_MAP_RAIN][0] + rb, L.c4) : 0.f;
+    s.drain = (d.flags & RIFT_DRAIN_GRID) ? ld(d.map[RIFT_MAP_DRAIN][0] + rb, L.c4) : 0.f;
     return s;
 }
 
@@ -210,21 +233,20 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
     r.b = raw.b;
     if constexpr (Stage == 2) {
         r.hraw = h;
-        r.hn = (h > kEps) ? h : 0.f;
         r.hun = hu;
         r.hvn = hv;
         if (pick(tile_row_slot(L, sr), L.act0, L.act1, L.act2)) {
             // Integrate_1_k (grid.cu:856-922) of this cell from U^n and dU;
             // G is recomputed from U^n exactly as the stage-1 flux forms it
             const float G = manning_g(raw.n, hu, hv, h, d.kappa);
-            const float hcl = r.hn;
+            const float hcl = (h > kEps) ? h : 0.f;
             float dh = raw.dh;
             if ((d.flags & RIFT_SURGE_GRID) && raw.sg != d.nodata && raw.sg > 0.f)
-                dh = (raw.sg - (hcl + dh * dt)) * inv_dt;
-            const float hnew = hcl + dh * dt;
-            const float rden = frcp(1.f - dt * G);
-            hu = (hu + raw.dhu * dt) * rden;
-            hv = (hv + raw.dhv * dt) * rden;
+                dh = (raw.sg - fmaf(dh, dt, hcl)) * inv_dt;
+            const float hnew = fmaf(dh, dt, hcl);
+            const float rden = frcp(fmaf(-dt, G, 1.f));
+            hu = fmaf(raw.dhu, dt, hu) * rden;
+            hv = fmaf(raw.dhv, dt, hv) * rden;
             h = (hnew > kEps) ? hnew : 0.f;
         }
     }
@@ -241,15 +263,20 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
     return r;
 }
 
-// Stage 2 holds two row windows (U* and U^n); four waves per SIMD keep it
-// under 128 VGPRs without spilling.
-template <int Stage, bool Textbook>
-__global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
+// Stage 2 holds two row windows (U* and U^n).  Waves = waves per SIMD the
+// register budget allows (launch variants).
+template <int Stage, bool Textbook, int Waves>
+__global__ __launch_bounds__(256, Waves) void k_march(Dev d)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wavefront index is uniform across the wave: in a scalar register,
+    // every unit and row index below is scalar too (SALU address math,
+    // scalar branches)
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = blockIdx.x * 4 + wid;
     const int nwaves = gridDim.x * 4;
-    const int nunits = d.sc->n_units;
+    // stage 1 runs before the dt kernel moves the unit count
+    const int nunits = min((Stage == 1) ? d.sc->n_units_acc : d.sc->n_units, d.nunits);
     const float dt = (float)d.sc->dt, inv_dt = 1.f / dt;
     const float dtp = (float)d.sc->dt_prev, inv_dtp = 1.f / dtp;
     const float kappa = d.kappa;
@@ -265,14 +292,14 @@ __global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
     // cache lines with a unit (its vertical and horizontal neighbours) run
     // on the same XCD at about the same time.  Otherwise the compacted list
     // is walked grid-stride.
-    const bool dense = nunits == d.nunits;
+    const bool dense = d.band && nunits == d.nunits;
     int kbeg, kend, kstep;
     if (dense) {
         // bands = min(8, workgroups): every band has at least one workgroup
         const int nb = min((int)gridDim.x, 8);
         const int x = (int)blockIdx.x % nb;
         const int nwg_x = ((int)gridDim.x - x + nb - 1) / nb;
-        kbeg = (int)((long long)d.nunits * x / nb) + ((int)blockIdx.x / nb) * 4 + (threadIdx.x >> 6);
+        kbeg = (int)((long long)d.nunits * x / nb) + ((int)blockIdx.x / nb) * 4 + wid;
         kend = (int)((long long)d.nunits * (x + 1) / nb);
         kstep = nwg_x * 4;
     } else {
@@ -282,13 +309,16 @@ __global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
     }
     (void)nwaves;
     for (int k = kbeg; k < kend; k += kstep) {
-        const int u = dense ? k : d.ulist[k];
+        const int u = __builtin_amdgcn_readfirstlane(dense ? k : d.ulist[k]);
         const int s = u % d.nstrips, rr = u / d.nstrips;
         Lane L;
-        L.c = kStrip * s + lane;
-        L.sc = min(max(L.c, 2), d.nx - 3);
-        L.gx = (L.c < 2) || (L.c > d.nx - 3);
-        L.txo = (L.sc - 2) / kTileW;
+        const int c = kStrip * s + lane;
+        const int sc = min(max(c, 2), d.nx - 3);
+        L.c4 = 4u * (unsigned)c;
+        L.sc4 = 4u * (unsigned)sc;
+        L.pl4 = 4u * (unsigned)d.soff;
+        L.gx = (c < 2) || (c > d.nx - 3);
+        L.txo = (sc - 2) / kTileW;
         L.r = rr;
         {
             const int Tm = max(rr - 1, 0) * d.nbx + L.txo, T0 = rr * d.nbx + L.txo;
@@ -296,7 +326,7 @@ __global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
             L.sel0 = d.sel[Tm] != 0; L.sel1 = d.sel[T0] != 0; L.sel2 = d.sel[Tp] != 0;
             L.act0 = d.act[Tm] != 0; L.act1 = d.act[T0] != 0; L.act2 = d.act[Tp] != 0;
         }
-        const bool out_lane = lane >= 2 && lane <= kStrip + 1 && L.c <= d.nx - 3;
+        const bool out_lane = lane >= 2 && lane <= kStrip + 1 && c <= d.nx - 3;
         const bool act_own = L.act1;
         const int j0 = rr * kTileH + 2;
         const int jend = min(j0 + kTileH, d.ny - 2);
@@ -342,13 +372,10 @@ __global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
             const float fw0 = from_left(fE.f0), fw1 = from_left(fE.f1), fw2 = from_left(fE.f2);
 
             if (out_lane && act_own) {
-                const size_t qc = cell(d, L.c, j);
-                float dh = -(fE.f0 - fw0) * inv_dx;
-                float dhu = -(fE.f1 - fw1) * inv_dx;
-                float dhv = -(fE.f2 - fw2) * inv_dx;
-                dh += -(gN.f0 - gS.f0) * inv_dy;
-                dhu += -(gN.f1 - gS.f1) * inv_dy;
-                dhv += -(gN.f2 - gS.f2) * inv_dy;
+                const size_t rbo = row_of(d, j);   // output row (uniform)
+                float dh = flux_divergence(fE.f0, fw0, gN.f0, gS.f0, inv_dx, inv_dy);
+                float dhu = flux_divergence(fE.f1, fw1, gN.f1, gS.f1, inv_dx, inv_dy);
+                float dhv = flux_divergence(fE.f2, fw2, gN.f2, gS.f2, inv_dx, inv_dy);
                 mxl = fmaxf(mxl, fE.speed);
                 myl = fmaxf(myl, gN.speed);
 
@@ -366,7 +393,7 @@ __global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
                     const float cap = hC * inv_dt;
                     S0 -= (src.drain > cap) ? cap : src.drain;
                 }
-                const bool srcc = (fl & RIFT_HYDROGRAPH) && L.c >= d.si0 && L.c <= d.si1 &&
+                const bool srcc = (fl & RIFT_HYDROGRAPH) && c >= d.si0 && c <= d.si1 &&
                                   j >= d.sj0 && j <= d.sj1;
                 if (srcc) S0 += src_rate;
                 float Fnew = 0.f;
@@ -377,38 +404,44 @@ __global__ __launch_bounds__(256, (Stage == 2 ? 4 : 5)) void k_march(Dev d)
                         // the stage-1 rate (previous dt, stored depth), then
                         // F* = F + dF dt of Integrate_1_k (grid.cu:917-923)
                         const float f1 = green_ampt(src.K, src.F, d.psi_dtheta, C.hraw, dtp, inv_dtp);
-                        const float Fs = src.F + f1 * dt;
+                        const float Fs = fmaf(f1, dt, src.F);
                         const float f2 = green_ampt(src.K, Fs, d.psi_dtheta, hC, dt, inv_dt);
                         S0 -= f2;
-                        Fnew = 0.5f * (src.F + Fs + f2 * dt);
+                        Fnew = 0.5f * fmaf(f2, dt, src.F + Fs);
                     }
                 }
                 dh += S0;
                 dhu += S1;
                 dhv += S2;
                 if constexpr (Stage == 1) {
-                    d.dh[qc] = dh;
-                    d.dhu[qc] = dhu;
-                    d.dhv[qc] = dhv;
+                    at(d.dh + rbo, L.c4) = dh;
+                    at(d.dhu + rbo, L.c4) = dhu;
+                    at(d.dhv + rbo, L.c4) = dhv;
                 } else {
                     // Integrate_2_k (grid.cu:1077-1123)
-                    const float hn = 0.5f * (C.hn + C.h + dh * dt);
-                    const float hun = 0.5f * (C.hun + C.hu + dhu * dt);
-                    const float hvn = 0.5f * (C.hvn + C.hv + dhv * dt);
+                    // U^n depth with the dry clamp of ComputeFluxes_k
+                    const float h_n = (C.hraw > kEps) ? C.hraw : 0.f;
+                    const float hn = 0.5f * fmaf(dh, dt, h_n + C.h);
+                    const float hun = 0.5f * fmaf(dhu, dt, C.hun + C.hu);
+                    const float hvn = 0.5f * fmaf(dhv, dt, C.hvn + C.hv);
                     const float hc = (hn > kEps) ? hn : 0.f;
-                    const size_t qo = qc + (L.sel1 ? 0 : (size_t)d.soff);
-                    d.h[qo] = hc;
-                    d.hu[qo] = hun;
-                    d.hv[qo] = hvn;
-                    if (fl & RIFT_INFILTRATION) d.F[qc] = Fnew;
+                    // into the buffer the tile does not read
+                    const unsigned lo = L.c4 + (L.sel1 ? 0u : L.pl4);
+                    at(d.h + rbo, lo) = hc;
+                    at(d.hu + rbo, lo) = hun;
+                    at(d.hv + rbo, lo) = hvn;
+                    if (fl & RIFT_INFILTRATION) at(d.F + rbo, L.c4) = Fnew;
                     if (fl & RIFT_SAVE_MAX) {
                         const float fv = vel_factor(hc, kappa);
                         const float uC = fv * hun, vC = fv * hvn;
-                        d.hmax[qc] = fmaxf(d.hmax[qc], hc);
-                        d.qmax[qc] = fmaxf(d.qmax[qc], hc * fsqrt(vC * vC + uC * uC));
+                        float& hm = at(d.hmax + rbo, L.c4);
+                        float& qm = at(d.qmax + rbo, L.c4);
+                        hm = fmaxf(hm, hc);
+                        qm = fmaxf(qm, hc * fsqrt(vC * vC + uC * uC));
                     }
                     if (fl & RIFT_SAVE_ARRIVAL) {
-                        if (d.twet[qc] < 0.f && hc > kEps) d.twet[qc] = tnow;
+                        float& tw = at(d.twet + rbo, L.c4);
+                        if (tw < 0.f && hc > kEps) tw = tnow;
                     }
                     bool w = hc > kEps || srcc;
                     if ((fl & RIFT_RAIN_AVG) && rain_rate > 0.f) w = true;
@@ -500,37 +533,96 @@ __global__ __launch_bounds__(256) void k_clear_sel(Dev d)
 
 // dt of the fused step: max over every tile of the stage-1 maxima (active
 // tiles) or the stage-2 maxima of the last step a tile was active (others),
-// in one workgroup, then ComputeTimestep's arithmetic.
-__global__ __launch_bounds__(1024) void k_timestep2(Dev d)
+// then ComputeTimestep's arithmetic.  The same pass counts the active tiles
+// and their interior cells.  Workgroups reduce grid-stride slices (all loads
+// independent, so they overlap) into device accumulators; the last workgroup
+// to finish forms dt, moves the unit count to stage 2, clears the
+// accumulators for the next step and evaluates the next step's forcing at
+// the new time (UpdateSource runs at the start of the next RunSimulation,
+// simulator.cu:780-784, with the t of this step).
+constexpr int kDtBlocks = 32;
+
+__global__ __launch_bounds__(256) void k_timestep2(Dev d)
 {
-    __shared__ float red[2][16];
+    __shared__ float red[2][4];
+    __shared__ unsigned long long redc[4];
+    __shared__ int redn[4];
+    __shared__ bool last;
     float vx = 0.f, vy = 0.f;
-    for (int t = threadIdx.x; t < d.ntiles; t += 1024) {
+    unsigned long long cells = 0ull;
+    int nact = 0;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < d.ntiles; t += gridDim.x * 256) {
         const bool a = d.act[t] != 0;
-        vx = fmaxf(vx, a ? d.mx1[t] : d.mx[t]);
-        vy = fmaxf(vy, a ? d.my1[t] : d.my[t]);
+        const float x1 = d.mx1[t], x0 = d.mx[t], y1 = d.my1[t], y0 = d.my[t];
+        vx = fmaxf(vx, a ? x1 : x0);
+        vy = fmaxf(vy, a ? y1 : y0);
+        if (a) {
+            const int bx = t % d.nbx, by = t / d.nbx;
+            const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
+            const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
+            if (i1 > i0 && j1 > j0) cells += (unsigned long long)((i1 - i0) * (j1 - j0));
+            ++nact;
+        }
     }
     vx = wave_max(vx);
     vy = wave_max(vy);
+    for (int off = 32; off > 0; off >>= 1) {
+        cells += __shfl_xor(cells, off);
+        nact += __shfl_xor(nact, off);
+    }
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        red[0][threadIdx.x >> 6] = vx;
-        red[1][threadIdx.x >> 6] = vy;
+        red[0][w] = vx;
+        red[1][w] = vy;
+        redc[w] = cells;
+        redn[w] = nact;
     }
     __syncthreads();
+    Scalars* sc = d.sc;
     if (threadIdx.x == 0) {
-        for (int w = 1; w < 16; ++w) {
-            vx = fmaxf(vx, red[0][w]);
-            vy = fmaxf(vy, red[1][w]);
+        for (int k = 1; k < 4; ++k) {
+            vx = fmaxf(vx, red[0][k]);
+            vy = fmaxf(vy, red[1][k]);
+            cells += redc[k];
+            nact += redn[k];
         }
-        vx = fmaxf(vx, red[0][0]);
-        vy = fmaxf(vy, red[1][0]);
-        finish_timestep(d, vx, vy);
+        atomicMax(&sc->mx_bits, __float_as_uint(vx));   // speeds are >= 0
+        atomicMax(&sc->my_bits, __float_as_uint(vy));
+        atomicAdd(&sc->wet_cells_acc, cells);
+        atomicAdd(&sc->n_active_acc, nact);
+        __threadfence();
+        last = atomicAdd(&sc->ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        const float vx_all = __uint_as_float(atomicAdd(&sc->mx_bits, 0u));
+        const float vy_all = __uint_as_float(atomicAdd(&sc->my_bits, 0u));
+        const unsigned long long cells_all = atomicAdd(&sc->wet_cells_acc, 0ull);
+        sc->n_active = atomicAdd(&sc->n_active_acc, 0);
+        sc->wet_cells = cells_all;
+        sc->wcu_total += cells_all;
+        finish_timestep(d, vx_all, vy_all);
+        sc->mx_bits = 0u;
+        sc->my_bits = 0u;
+        sc->wet_cells_acc = 0ull;
+        sc->n_active_acc = 0;
+        sc->ticket = 0u;
+        sc->n_units = sc->n_units_acc;
+        sc->n_units_acc = 0;
+        const Forcing f = forcing_at(d, sc);
+        sc->nxt_rain = f.rain;
+        sc->nxt_drain = f.drain;
+        sc->nxt_hydro = f.hydro;
+        sc->nxt_cell = f.cell;
+        sc->next_valid = 1;
     }
 }
 
 void launch_timestep2(const Dev& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_timestep2, dim3(1), dim3(1024), 0, s, d);
+    const int blocks = std::max(1, std::min(kDtBlocks, (d.ntiles + 255) / 256));
+    hipLaunchKernelGGL(k_timestep2, dim3(blocks), dim3(256), 0, s, d);
 }
 
 void launch_grow2(const Dev& d, hipStream_t s)
@@ -539,18 +631,30 @@ void launch_grow2(const Dev& d, hipStream_t s)
     hipLaunchKernelGGL(k_grow2, dim3((n + 255) / 256), dim3(256), 0, s, d);
 }
 
-void launch_march(const Dev& d, int stage, hipStream_t s)
+// Launch variants (all give identical results; they differ in register
+// budget and work order): bit 0 = stage 1 at 4 instead of 5 waves per SIMD,
+// bit 1 = stage 2 at 3 instead of 4 waves per SIMD; the XCD banding of the
+// work order is Dev::band.
+template <int Stage, int Waves>
+static void launch_march_w(const Dev& d, int blocks, hipStream_t s)
+{
+    if (d.flags & RIFT_TEXTBOOK)
+        hipLaunchKernelGGL((k_march<Stage, true, Waves>), dim3(blocks), dim3(256), 0, s, d);
+    else
+        hipLaunchKernelGGL((k_march<Stage, false, Waves>), dim3(blocks), dim3(256), 0, s, d);
+}
+
+void launch_march(const Dev& d, int stage, hipStream_t s, int variant)
 {
     int blocks = (d.nunits + 3) / 4;
     if (blocks > 2048) blocks = 2048;
     if (blocks < 1) blocks = 1;
-    const bool tb = d.flags & RIFT_TEXTBOOK;
     if (stage == 1) {
-        if (tb) hipLaunchKernelGGL((k_march<1, true>), dim3(blocks), dim3(256), 0, s, d);
-        else hipLaunchKernelGGL((k_march<1, false>), dim3(blocks), dim3(256), 0, s, d);
+        if (variant & 1) launch_march_w<1, 4>(d, blocks, s);
+        else launch_march_w<1, 5>(d, blocks, s);
     } else {
-        if (tb) hipLaunchKernelGGL((k_march<2, true>), dim3(blocks), dim3(256), 0, s, d);
-        else hipLaunchKernelGGL((k_march<2, false>), dim3(blocks), dim3(256), 0, s, d);
+        if (variant & 2) launch_march_w<2, 3>(d, blocks, s);
+        else launch_march_w<2, 4>(d, blocks, s);
     }
 }
 

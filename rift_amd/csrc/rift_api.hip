@@ -55,12 +55,19 @@ struct rift_grid {
     int R = 0;                 // padded rows
     size_t plane = 0;          // P*R floats
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_sizes;   // bytes of each allocation (same order)
     std::vector<double> bc_ref;   // reference BC (InitGrid_k rule), host copy
     std::vector<double> n_host;
     double* dsum = nullptr;       // device scratch for FP64 sums
     int grid_tiles = 0;
     bool static_loaded = false, inited = false;
     bool scattered = false;       // fused steps left per-tile buffer selects
+    // the device holds this step's forcing (or the next step's, evaluated by
+    // the last dt kernel) and zero unit counters: a fused step needs no
+    // begin_step launch; anything that changes time, series, sources or the
+    // counters clears it
+    bool primed = false;
+    int variant = 0;              // launch variant of the marching kernels
     bool per_phase = false;       // rift_step uses the one-kernel-per-phase path
     // captured fused steps (1 and kGraphSteps steps); the kernels take the
     // handle's Dev by value, so any change of it invalidates the graphs
@@ -86,6 +93,7 @@ float* alloc_plane(rift_grid* g)
     if (hipMalloc(&p, g->plane * sizeof(float)) != hipSuccess) return nullptr;
     if (hipMemset(p, 0, g->plane * sizeof(float)) != hipSuccess) return nullptr;
     g->allocs.push_back(p);
+    g->alloc_sizes.push_back(g->plane * sizeof(float));
     return static_cast<float*>(p);
 }
 
@@ -96,6 +104,7 @@ T* alloc_bytes(rift_grid* g, size_t n)
     if (hipMalloc(&p, n * sizeof(T) + 64) != hipSuccess) return nullptr;
     if (hipMemset(p, 0, n * sizeof(T) + 64) != hipSuccess) return nullptr;
     g->allocs.push_back(p);
+    g->alloc_sizes.push_back(n * sizeof(T) + 64);
     return static_cast<T*>(p);
 }
 
@@ -199,6 +208,12 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.P = ((std::max(kTileW * d.nbx + 4, kStrip * d.nstrips + 4) + 63) / 64) * 64;
     g->R = kTileH * d.nby + 4;
     g->plane = (size_t)d.P * (size_t)g->R;
+    // the marching kernels address a cell of either state buffer by a 32-bit
+    // byte offset from its row (column + one plane): a plane stays < 2^30
+    if (g->plane + (size_t)d.P >= ((size_t)1 << 30)) {
+        delete g;
+        return fail(RIFT_E_UNSUPPORTED, "rift_create: domain too large (a field plane must hold < 2^30 cells)");
+    }
     d.dx = desc->dx;
     d.dy = desc->dy;
     d.inv_dx = (float)(1.0 / desc->dx);
@@ -212,6 +227,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.si0 = d.sj0 = 1;
     d.si1 = d.sj1 = 0;
     d.nsrc = 0;
+    d.band = 1;
 
     hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -221,18 +237,22 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     const unsigned f = desc->flags;
     bool ok = true;
     auto P = [&](float*& p) { p = alloc_plane(g); ok = ok && p; };
-    // the two state buffers are six consecutive planes, so a tile's buffer
-    // select is one element offset (d.soff) added to the h/hu/hv indices
+    // the two state buffers are six consecutive planes, each field followed
+    // by its second buffer (h, h0, hu, hu0, hv, hv0), so a tile's buffer
+    // select is one plane (d.soff elements) added to the h/hu/hv indices and
+    // fits the 32-bit lane offsets of the marching kernels
     {
         void* p = nullptr;
         ok = hipMalloc(&p, 6 * g->plane * sizeof(float)) == hipSuccess &&
              hipMemset(p, 0, 6 * g->plane * sizeof(float)) == hipSuccess;
         if (ok) {
             g->allocs.push_back(p);
+            g->alloc_sizes.push_back(6 * g->plane * sizeof(float));
             float* b = static_cast<float*>(p);
-            d.h = b; d.hu = b + g->plane; d.hv = b + 2 * g->plane;
-            d.h0 = b + 3 * g->plane; d.hu0 = b + 4 * g->plane; d.hv0 = b + 5 * g->plane;
-            d.soff = (long long)(3 * g->plane);
+            d.h = b; d.h0 = b + g->plane;
+            d.hu = b + 2 * g->plane; d.hu0 = b + 3 * g->plane;
+            d.hv = b + 4 * g->plane; d.hv0 = b + 5 * g->plane;
+            d.soff = (long long)g->plane;
         }
     }
     P(d.dh); P(d.dhu); P(d.dhv); P(d.G); P(d.n); P(d.BC); P(d.BX); P(d.BY);
@@ -383,6 +403,7 @@ int rift_init(rift_grid* g, double t0)
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(g->stream));
     g->inited = true;
+    g->primed = false;
     return RIFT_OK;
 }
 
@@ -397,6 +418,7 @@ int rift_set_series(rift_grid* g, int which, const float* time_s, const float* v
     std::memcpy(tab[0].rate, value, n * sizeof(float));
     HIPCHK(hipMemcpyAsync(g->d.ser + which, tab.data(), sizeof(SeriesTable), hipMemcpyHostToDevice,
                           g->stream));
+    g->primed = false;
     if (which == RIFT_SER_HYDROGRAPH) {
         int zero = 0;
         HIPCHK(hipMemcpyAsync(&g->d.sc->hydro_external, &zero, sizeof(int), hipMemcpyHostToDevice,
@@ -410,6 +432,7 @@ int rift_set_source_cells(rift_grid* g, const int* idx, long n)
 {
     if (!g || (n > 0 && !idx)) return fail(RIFT_E_ARG, "rift_set_source_cells: bad argument");
     Dev& d = g->d;
+    g->primed = false;
     if (n == 0) {
         d.si0 = d.sj0 = 1;
         d.si1 = d.sj1 = 0;
@@ -441,6 +464,7 @@ int rift_set_source_value(rift_grid* g, double value)
 {
     if (!g) return fail(RIFT_E_ARG, "rift_set_source_value: null handle");
     struct { int ext; int pad; double v; } s{1, 0, value};
+    g->primed = false;
     HIPCHK(hipMemcpyAsync(&g->d.sc->hydro_external, &s.ext, sizeof(int), hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipMemcpyAsync(&g->d.sc->hydro_value, &s.v, sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -488,6 +512,7 @@ static int ensure_canonical(rift_grid* g)
     do {                                                                    \
         if (!(g)) return fail(RIFT_E_ARG, "null handle");                   \
         if (!(g)->inited) return fail(RIFT_E_STATE, "rift_init not called"); \
+        (g)->primed = false;                                                \
         int rc_ = ensure_canonical(g);                                      \
         if (rc_) return rc_;                                                \
     } while (0)
@@ -570,12 +595,22 @@ static int capture_steps(rift_grid* g, int n, hipGraphExec_t* out)
     return RIFT_OK;
 }
 
-// The fused step: forcing, tile/unit compaction, stage-1 march, dt,
-// stage-2 march (Integrate_1 + flux + Heun + diagnostics in one pass).
+// Forcing and counters for a fused step when the last dt kernel did not
+// prepare them.
+static void prime(rift_grid* g)
+{
+    if (g->primed) return;
+    launch_begin_step(g->d, g->stream);
+    g->primed = true;
+}
+
+// The fused step: tile/unit compaction (installing the forcing), stage-1
+// march, dt (+ the next step's forcing), stage-2 march (Integrate_1 + flux +
+// Heun + diagnostics in one pass).  Runs primed, so that it can be captured.
 static int step_fused(rift_grid* g)
 {
     Dev& d = g->d;
-    launch_begin_step(d, g->stream);   // forcing + counter reset
+    prime(g);
     {
         ProfScope ps(g, 4);
         if (d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(d, g->stream);
@@ -583,7 +618,7 @@ static int step_fused(rift_grid* g)
     }
     {
         ProfScope ps(g, 0);
-        launch_march(d, 1, g->stream);
+        launch_march(d, 1, g->stream, g->variant);
     }
     {
         ProfScope ps(g, 5);
@@ -591,7 +626,7 @@ static int step_fused(rift_grid* g)
     }
     {
         ProfScope ps(g, 1);
-        launch_march(d, 2, g->stream);
+        launch_march(d, 2, g->stream, g->variant);
     }
     g->scattered = true;
     return check_launch("fused step");
@@ -613,6 +648,7 @@ int rift_step(rift_grid* g, int nsteps)
         }
         int rc;
         int left = nsteps;
+        prime(g);   // before capture: graphs hold the primed step only
         if (left >= kGraphSteps && !g->graphN && (rc = capture_steps(g, kGraphSteps, &g->graphN)))
             return rc;
         for (; left >= kGraphSteps; left -= kGraphSteps) HIPCHK(hipGraphLaunch(g->graphN, g->stream));
@@ -655,9 +691,86 @@ int rift_set_option(rift_grid* g, int option, int value)
     case RIFT_OPT_GRAPHS:
         g->use_graphs = value != 0;
         return RIFT_OK;
+    case RIFT_OPT_VARIANT:
+        if (value < 0 || value >= RIFT_NVARIANTS) return fail(RIFT_E_ARG, "rift_set_option: no such variant");
+        drop_graphs(g);   // the graphs hold the launch configuration
+        g->variant = value & (kMarchVariants - 1);
+        g->d.band = (value & kMarchVariants) ? 0 : 1;
+        return RIFT_OK;
     default:
         return fail(RIFT_E_ARG, "rift_set_option: unknown option");
     }
+}
+
+// Device state of g copied into c, a handle created from the same
+// descriptor (same allocations in the same order), with the host-side state
+// that goes with it.
+static int copy_state(rift_grid* g, rift_grid* c)
+{
+    if (c->allocs.size() != g->allocs.size()) return fail(RIFT_E_STATE, "copy_state: layouts differ");
+    HIPCHK(hipStreamSynchronize(g->stream));
+    for (size_t k = 0; k < g->allocs.size(); ++k) {
+        if (c->alloc_sizes[k] != g->alloc_sizes[k]) return fail(RIFT_E_STATE, "copy_state: layouts differ");
+        HIPCHK(hipMemcpyAsync(c->allocs[k], g->allocs[k], g->alloc_sizes[k], hipMemcpyDeviceToDevice,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    Dev& cd = c->d;
+    const Dev& gd = g->d;
+    cd.si0 = gd.si0; cd.si1 = gd.si1; cd.sj0 = gd.sj0; cd.sj1 = gd.sj1;
+    cd.nsrc = gd.nsrc;
+    cd.n_uniform = gd.n_uniform;
+    cd.n_const = gd.n_const;
+    c->bc_ref = g->bc_ref;
+    c->n_host = g->n_host;
+    c->static_loaded = g->static_loaded;
+    c->inited = g->inited;
+    c->scattered = g->scattered;
+    c->primed = g->primed;
+    return RIFT_OK;
+}
+
+int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
+{
+    if (!g || !best || nsteps < 1) return fail(RIFT_E_ARG, "rift_autotune: bad argument");
+    if (!g->inited) return fail(RIFT_E_STATE, "rift_init not called");
+    rift_grid* c = nullptr;
+    int rc = rift_create(&g->desc, &c);
+    if (rc) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double t_best = 0.0;
+    int v_best = -1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(RIFT_E_HIP, "rift_autotune: hipEventCreate failed");
+    }
+    for (int v = 0; v < RIFT_NVARIANTS && rc == RIFT_OK; ++v) {
+        // warm-up from the snapshot (captures the variant's graphs), then
+        // the timed run from the same snapshot
+        if ((rc = copy_state(g, c)) || (rc = rift_set_option(c, RIFT_OPT_VARIANT, v))) break;
+        c->per_phase = g->per_phase;
+        if ((rc = rift_step(c, nsteps)) || (rc = copy_state(g, c))) break;
+        if ((rc = rift_step(c, 2))) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(RIFT_E_HIP, "hipEventRecord"); break; }
+        if ((rc = rift_step(c, nsteps))) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(RIFT_E_HIP, "rift_autotune: event timing failed");
+            break;
+        }
+        const double per = (double)ms / nsteps;
+        if (ms_per_step) ms_per_step[v] = per;
+        if (v_best < 0 || per < t_best) {
+            t_best = per;
+            v_best = v;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    rift_destroy(c);
+    if (rc) return rc;
+    *best = v_best;
+    return rift_set_option(g, RIFT_OPT_VARIANT, v_best);
 }
 
 int rift_sync(rift_grid* g)
@@ -681,6 +794,7 @@ int rift_get_time(rift_grid* g, double* t, double* dt)
 int rift_set_time(rift_grid* g, double t, double dt)
 {
     if (!g) return fail(RIFT_E_ARG, "rift_set_time: null handle");
+    g->primed = false;
     double v[2] = {t, dt};
     HIPCHK(hipMemcpyAsync(g->d.sc, v, 2 * sizeof(double), hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));

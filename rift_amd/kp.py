@@ -35,6 +35,8 @@ FIELDS = ["w", "hu", "hv", "depth", "bc", "n", "K", "F", "dw", "dhu", "dhv", "G"
           "h_max", "q_max", "t_wet", "t_peak", "t_dry", "surge", "rainmap", "drainmap", "q"]
 SER_HYETO, SER_DRAIN, SER_HYDRO = 0, 1, 2
 MAP_RAIN, MAP_DRAIN, MAP_SURGE = 0, 1, 2
+OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT = 1, 2, 3
+NVARIANTS = 8
 KERNEL_CLASSES = ["flux1", "stage2_flux", "integrate", "boundaries", "grow", "timestep"]
 
 # (name, restype, argtypes) of every exported entry point
@@ -73,6 +75,7 @@ SIGNATURES = [
     ("rift_integrate_2", C.c_int, [_VP]),
     ("rift_apply_boundaries", C.c_int, [_VP]),
     ("rift_set_option", C.c_int, [_VP, C.c_int, C.c_int]),
+    ("rift_autotune", C.c_int, [_VP, C.c_int, _I, _D]),
     ("rift_sync", C.c_int, [_VP]),
     ("rift_get_time", C.c_int, [_VP, _D, _D]),
     ("rift_set_time", C.c_int, [_VP, C.c_double, C.c_double]),
@@ -262,6 +265,18 @@ class Grid:
     def set_graphs(self, on):
         """Replay the fused step from captured HIP graphs (default on)."""
         _check(load().rift_set_option(self.h, 2, 1 if on else 0), "rift_set_option")
+
+    def set_variant(self, v):
+        """Launch variant of the fused step's marching kernels (0..NVARIANTS-1)."""
+        _check(load().rift_set_option(self.h, OPT_VARIANT, int(v)), "rift_set_option")
+
+    def autotune(self, nsteps=16):
+        """Times every launch variant on a copy of the current state, selects
+        the fastest; returns (best, ms_per_step of each variant)."""
+        best = C.c_int()
+        ms = np.zeros(NVARIANTS)
+        _check(load().rift_autotune(self.h, int(nsteps), C.byref(best), _dp(ms)), "rift_autotune")
+        return best.value, ms
 
     def sync(self):
         _check(load().rift_sync(self.h), "rift_sync")

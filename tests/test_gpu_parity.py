@@ -244,3 +244,38 @@ def test_graph_replays_match_eager_launches(rift):
         np.testing.assert_array_equal(a.tiles()[0], b.tiles()[0])
     a.close()
     b.close()
+
+
+def test_launch_variants_bit_identical_and_autotuner(rift):
+    """Every launch variant of the fused step (register budget per stage, work
+    order) gives the default variant's state, dt and wet set bit for bit; the
+    autotuner times them on a copy and leaves the handle's state untouched."""
+    cfg = synth.rain_basin(160, seed=11)
+    kw = dict(n_const=cfg["n_const"], K=cfg["K"], psi=cfg["psi"], dtheta=cfg["dtheta"],
+              flags=rift.TILE_ACCEL | rift.RAIN_AVG)
+    t, v = synth.series_hours_to_seconds(cfg["hyeto_hours"], cfg["hyeto_mmhr"])
+    ref = None
+    for var in range(rift.NVARIANTS):
+        g = rift.grid_from_rasters(cfg["dem"], cfg["dx"], **kw)
+        g.set_series(rift.SER_HYETO, t, v)
+        g.set_variant(var)
+        g.step(40)
+        got = (g.time, [g.field(f) for f in ("depth", "hu", "hv", "F")], g.tiles()[0])
+        if ref is None:
+            ref = got
+        else:
+            assert got[0] == ref[0], var
+            for a, b in zip(got[1], ref[1]):
+                np.testing.assert_array_equal(a, b)
+            np.testing.assert_array_equal(got[2], ref[2])
+        g.close()
+    g = rift.grid_from_rasters(cfg["dem"], cfg["dx"], **kw)
+    g.set_series(rift.SER_HYETO, t, v)
+    g.step(20)
+    before = (g.time, g.field("depth"))
+    best, ms = g.autotune(16)
+    assert 0 <= best < rift.NVARIANTS and (ms > 0).all() and ms[best] == ms.min()
+    assert g.time == before[0]
+    np.testing.assert_array_equal(g.field("depth"), before[1])
+    g.step(20)   # continues with the chosen variant
+    g.close()
