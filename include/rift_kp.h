@@ -167,10 +167,12 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
 /* RIFT_OPT_VARIANT: launch variant of the fused step's marching kernels,
  * 0 .. RIFT_NVARIANTS-1 (default 0): bit 0 = stage 1 with a larger register
  * budget (4 instead of 5 waves per SIMD), bit 1 = stage 2 likewise (3
- * instead of 4), bit 2 = plain grid-stride work order instead of the
- * XCD-banded one.  Every variant computes the same results bit for bit. */
+ * instead of 4), bit 2 = work shares dealt to workgroups in launch order
+ * instead of by XCD, bit 3 = shares of at least 7 rows instead of 14 (more
+ * waves on sparse steps).  Every variant computes the same results bit for
+ * bit. */
 #define RIFT_OPT_VARIANT 3
-#define RIFT_NVARIANTS 8
+#define RIFT_NVARIANTS 16
 int rift_set_option(rift_grid* g, int option, int value);
 
 /* Autotuner (no counterpart in the reference, whose 16 x 14 launch shape is

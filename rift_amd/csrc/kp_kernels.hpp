@@ -62,16 +62,15 @@ struct Dev {
     unsigned char *wet, *act;
     int* list;
     // fused step: per-tile buffer select (state of the tile's cells lives in
-    // h/hu/hv when 0, in h0/hu0/hv0 when 1), pending flips, the stage-2
-    // contributor word (count << 8 | wet contributors), the active units
+    // h/hu/hv when 0, in h0/hu0/hv0 when 1), pending flips, the active units
     unsigned char *sel, *flip;
-    unsigned* pending;
     int* ulist;
     int nstrips, nurows, nunits;
     long long soff;              // h0 - h (elements): offset of the second buffer
     int n_uniform;
     float n_const;
-    int band;                    // XCD-banded order of dense unit lists
+    int band;                    // XCD-banded order of the marching shares
+    int min_rows;                // least rows per marching share
     float *mx, *my;              // per-tile maxima of the last flux pass
     float *mx1, *my1;            // fused step: stage-1 maxima of this step
     Scalars* sc;
@@ -177,7 +176,9 @@ void launch_sum_interior(const Dev& d, const float* x, int mode, double* out, hi
 constexpr int kStrip = 60;
 void launch_grow2(const Dev& d, hipStream_t s);
 void launch_timestep2(const Dev& d, hipStream_t s);
-void launch_march(const Dev& d, int stage, hipStream_t s, int variant);
+void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks);
+// workgroups of a marching kernel resident at once on a device of `cus` CUs
+int march_resident_blocks(int stage, int variant, bool textbook, int cus);
 constexpr int kMarchVariants = 4;
 void launch_canonicalize(const Dev& d, hipStream_t s);
 

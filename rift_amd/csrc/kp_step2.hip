@@ -1,7 +1,7 @@
 // kp_step2.hip -- the fused Heun step: two marching kernels per step.
 //
 //   grow2      tile dilation (+ pending buffer flips, per-tile resets) and
-//              compaction of the active marching units
+//              compaction of the active marching units (strip-major)
 //   march<1>   stage-1 flux + sources of U^n            -> dU, tile speeds
 //   timestep   dt from all tile speeds                   (kp_kernels.hip)
 //   march<2>   U* = Integrate_1(U^n, dU) on the fly, flux + sources of U*,
@@ -9,10 +9,12 @@
 //              tile wet flags                            -> U^{n+1} in the
 //              other buffer of the tile
 //
-// A unit is one wavefront: lane L holds column c = 60*s + L of strip s and
-// marches the 18 rows j0-2 .. j0+15 of tile row r, keeping three rows in
-// registers; x neighbours come from DPP wavefront shifts, so no LDS is used.
-// Lanes 2..61 produce output columns 60s+2 .. 60s+61.
+// A wavefront marches down a strip: lane L holds column c = 60*s + L of
+// strip s, keeping a window of three rows in registers (two rows of halo
+// above and below the rows it outputs); x neighbours come from DPP
+// wavefront shifts, so no LDS is used.  Lanes 2..61 produce output columns
+// 60s+2 .. 60s+61.  A unit is one strip by one 14-row tile row; each wave
+// marches an equal share of the active units' rows (see k_march).
 //
 // Ghost cells are never stored: a lane or row outside the interior loads the
 // interior cell the reference's ApplyBoundaries copies from (grid.cu:1325-
@@ -97,22 +99,16 @@ __global__ __launch_bounds__(256) void k_grow2(Dev d)
         const bool a = dilated(d, t);
         d.act[t] = a ? 1 : 0;
         if (a) {
-            const int bx = t % d.nbx, by = t / d.nbx;
-            const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
-            const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
-            unsigned units = 0;
-            if (i1 > i0 && j1 > j0) units = (unsigned)((i1 - 1 - 2) / kStrip - (i0 - 2) / kStrip + 1);
             d.mx[t] = 0.f;
             d.my[t] = 0.f;
             d.mx1[t] = 0.f;
             d.my1[t] = 0.f;
-            // contributor count in bits 8.., wet contributors in bits 0..7
-            d.pending[t] = units << 8;
         }
     }
     if (is_unit) {
+        // units are numbered strip-major: u = strip * nurows + tile row
         const int u = t - d.ntiles;
-        const int s = u % d.nstrips, r = u / d.nstrips;
+        const int s = u / d.nurows, r = u - s * d.nurows;
         int txa, txb;
         strip_tiles(d, s, txa, txb);
         for (int tx = txa; tx <= txb; ++tx) ua |= dilated(d, r * d.nbx + tx);
@@ -263,8 +259,302 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
     return r;
 }
 
-// Stage 2 holds two row windows (U* and U^n).  Waves = waves per SIMD the
-// register budget allows (launch variants).
+// Lane constants for strip s, tile row rr.
+__device__ __forceinline__ Lane make_lane(const Dev& d, int s, int rr, int lane)
+{
+    Lane L;
+    const int c = kStrip * s + lane;
+    const int sc = min(max(c, 2), d.nx - 3);
+    L.c4 = 4u * (unsigned)c;
+    L.sc4 = 4u * (unsigned)sc;
+    L.pl4 = 4u * (unsigned)d.soff;
+    L.gx = (c < 2) || (c > d.nx - 3);
+    L.txo = (sc - 2) / kTileW;
+    L.r = rr;
+    const int Tm = max(rr - 1, 0) * d.nbx + L.txo, T0 = rr * d.nbx + L.txo;
+    const int Tp = min(rr + 1, d.nby - 1) * d.nbx + L.txo;
+    L.sel0 = d.sel[Tm] != 0; L.sel1 = d.sel[T0] != 0; L.sel2 = d.sel[Tp] != 0;
+    L.act0 = d.act[Tm] != 0; L.act1 = d.act[T0] != 0; L.act2 = d.act[Tp] != 0;
+    return L;
+}
+
+// Move the lane's tile-row window to tile row rr = L.r + 1 (down) or - 1.
+template <bool Up>
+__device__ __forceinline__ void shift_lane(const Dev& d, Lane& L, int rr)
+{
+    L.r = rr;
+    if constexpr (!Up) {
+        L.sel0 = L.sel1; L.sel1 = L.sel2;
+        L.act0 = L.act1; L.act1 = L.act2;
+        const int Tp = min(rr + 1, d.nby - 1) * d.nbx + L.txo;
+        L.sel2 = d.sel[Tp] != 0;
+        L.act2 = d.act[Tp] != 0;
+    } else {
+        L.sel2 = L.sel1; L.sel1 = L.sel0;
+        L.act2 = L.act1; L.act1 = L.act0;
+        const int Tm = max(rr - 1, 0) * d.nbx + L.txo;
+        L.sel0 = d.sel[Tm] != 0;
+        L.act0 = d.act[Tm] != 0;
+    }
+}
+
+// Scalars of a marching launch.
+struct StepScalars {
+    float dt, inv_dt, dtp, inv_dtp, kappa, rain_rate, drain_rate, src_rate, tnow, inv_dx, inv_dy;
+    unsigned fl;
+};
+
+// Per-tile results of the rows of tile row rr a wave marched in strip s:
+// speed maxima (stage 1 into mx1/my1 for this step's dt, stage 2 into mx/my,
+// which later steps see while the tile is inactive) and the tile wet flags
+// of Integrate_2_k, written by thread (0,0) of a tile, i.e. only for a tile
+// whose origin cell is interior (grid.cu:1160-1170): stage 1 resets the flag
+// of an active tile (to 1 without tile acceleration), stage 2 sets it where
+// a cell is wet.  Several waves may share a tile; every write is idempotent.
+template <int Stage>
+__device__ __forceinline__ void flush_tiles(const Dev& d, int s, int rr, const Lane& L, bool out_lane,
+                                            float mxl, float myl, bool wetl, int lane)
+{
+    float* const tmx = (Stage == 1) ? d.mx1 : d.mx;
+    float* const tmy = (Stage == 1) ? d.my1 : d.my;
+    int txa, txb;
+    strip_tiles(d, s, txa, txb);
+    for (int tx = txa; tx <= txb; ++tx) {
+        const int T = rr * d.nbx + tx;
+        if (!d.act[T]) continue;   // uniform
+        const bool seg = out_lane && L.txo == tx;
+        const float m1 = wave_max(seg ? mxl : 0.f);
+        const float m2 = wave_max(seg ? myl : 0.f);
+        const bool anyw = __ballot(seg && wetl) != 0ull;
+        if (lane == 0) {
+            atomicMax(reinterpret_cast<unsigned*>(tmx) + T, __float_as_uint(m1));
+            atomicMax(reinterpret_cast<unsigned*>(tmy) + T, __float_as_uint(m2));
+            const bool origin = tx * kTileW + 2 <= d.nx - 3 && rr * kTileH + 2 <= d.ny - 3;
+            if constexpr (Stage == 1) {
+                if (origin) d.wet[T] = (d.flags & RIFT_TILE_ACCEL) ? 0 : 1;
+            } else {
+                d.flip[T] = 1;
+                if (origin && anyw) d.wet[T] = 1;
+            }
+        }
+    }
+}
+
+// Sources, the stage update and the stores of output row j (lanes of active
+// tiles); returns the cell's wet flag (stage 2).
+template <int Stage>
+__device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, const Lane& L, int c,
+                                           int j, const Row& C, const Src& src, float dh, float dhu,
+                                           float dhv, float bxw, float bxe, float bys, float byn)
+{
+    const unsigned fl = S.fl;
+    const size_t rbo = row_of(d, j);   // output row (uniform)
+    const float hC = C.h;
+    const float S1 = -kG * hC * ((bxe - bxw) * S.inv_dx);
+    const float S2 = -kG * hC * ((byn - bys) * S.inv_dy);
+    float S0 = 0.f;
+    if (fl & RIFT_RAIN_AVG) S0 += S.rain_rate;
+    if (fl & RIFT_RAIN_GRID) S0 += src.rain;
+    if (fl & RIFT_DRAIN_AVG) {
+        const float cap = hC * S.inv_dt;
+        S0 -= (S.drain_rate > cap) ? cap : S.drain_rate;
+    }
+    if (fl & RIFT_DRAIN_GRID) {
+        const float cap = hC * S.inv_dt;
+        S0 -= (src.drain > cap) ? cap : src.drain;
+    }
+    const bool srcc = (fl & RIFT_HYDROGRAPH) && c >= d.si0 && c <= d.si1 && j >= d.sj0 && j <= d.sj1;
+    if (srcc) S0 += S.src_rate;
+    float Fnew = 0.f;
+    if (fl & RIFT_INFILTRATION) {
+        if constexpr (Stage == 1) {
+            S0 -= green_ampt(src.K, src.F, d.psi_dtheta, hC, S.dt, S.inv_dt);
+        } else {
+            // the stage-1 rate (previous dt, stored depth), then
+            // F* = F + dF dt of Integrate_1_k (grid.cu:917-923)
+            const float f1 = green_ampt(src.K, src.F, d.psi_dtheta, C.hraw, S.dtp, S.inv_dtp);
+            const float Fs = fmaf(f1, S.dt, src.F);
+            const float f2 = green_ampt(src.K, Fs, d.psi_dtheta, hC, S.dt, S.inv_dt);
+            S0 -= f2;
+            Fnew = 0.5f * fmaf(f2, S.dt, src.F + Fs);
+        }
+    }
+    dh += S0;
+    dhu += S1;
+    dhv += S2;
+    if constexpr (Stage == 1) {
+        at(d.dh + rbo, L.c4) = dh;
+        at(d.dhu + rbo, L.c4) = dhu;
+        at(d.dhv + rbo, L.c4) = dhv;
+        return false;
+    } else {
+        // Integrate_2_k (grid.cu:1077-1123); U^n depth with the dry clamp of
+        // ComputeFluxes_k
+        const float h_n = (C.hraw > kEps) ? C.hraw : 0.f;
+        const float hn = 0.5f * fmaf(dh, S.dt, h_n + C.h);
+        const float hun = 0.5f * fmaf(dhu, S.dt, C.hun + C.hu);
+        const float hvn = 0.5f * fmaf(dhv, S.dt, C.hvn + C.hv);
+        const float hc = (hn > kEps) ? hn : 0.f;
+        // into the buffer the tile does not read
+        const unsigned lo = L.c4 + (L.sel1 ? 0u : L.pl4);
+        at(d.h + rbo, lo) = hc;
+        at(d.hu + rbo, lo) = hun;
+        at(d.hv + rbo, lo) = hvn;
+        if (fl & RIFT_INFILTRATION) at(d.F + rbo, L.c4) = Fnew;
+        if (fl & RIFT_SAVE_MAX) {
+            const float fv = vel_factor(hc, S.kappa);
+            const float uC = fv * hun, vC = fv * hvn;
+            float& hm = at(d.hmax + rbo, L.c4);
+            float& qm = at(d.qmax + rbo, L.c4);
+            hm = fmaxf(hm, hc);
+            qm = fmaxf(qm, hc * fsqrt(vC * vC + uC * uC));
+        }
+        if (fl & RIFT_SAVE_ARRIVAL) {
+            float& tw = at(d.twet + rbo, L.c4);
+            if (tw < 0.f && hc > kEps) tw = S.tnow;
+        }
+        bool w = hc > kEps || srcc;
+        if ((fl & RIFT_RAIN_AVG) && S.rain_rate > 0.f) w = true;
+        if ((fl & RIFT_RAIN_GRID) && src.rain > 0.f) w = true;
+        return w;
+    }
+}
+
+// One run of a wave: output rows [ja, jb) of strip s, marched downwards
+// (increasing j) or upwards.  The window holds the output row C, the next
+// row D (in marching direction) and the raw row after it; the y flux of the
+// face behind C is carried from the previous row.  Crossing into another
+// tile row flushes the per-tile results and shifts the lane's tile-row
+// window.
+template <int Stage, bool Textbook, bool Up>
+__device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane)
+{
+    constexpr int step = Up ? -1 : 1;
+    const int c = kStrip * s + lane;
+    const bool out_lane = lane >= 2 && lane <= kStrip + 1 && c <= d.nx - 3;
+    int j = Up ? jb - 1 : ja;
+    const int jstop = Up ? ja - 1 : jb;
+    int rr = (j - 2) / kTileH;
+    Lane L = make_lane(d, s, rr, lane);
+    const float dt = S.dt, inv_dt = S.inv_dt, kappa = S.kappa;
+
+    // prologue: rows j-2 .. j+2 (both directions), by = BY of a row = the bed
+    // of its south face
+    const Row A = finish<Stage>(d, L, fetch<Stage>(d, L, j - 2 * step), j - 2 * step, dt, inv_dt);
+    const Raw rB = fetch<Stage>(d, L, j - step);
+    const Raw rC = fetch<Stage>(d, L, j);
+    const Raw rD = fetch<Stage>(d, L, j + step);
+    Raw rE = fetch<Stage>(d, L, j + 2 * step);
+    Src src = fetch_src(d, L, j);
+    const Row B = finish<Stage>(d, L, rB, j - step, dt, inv_dt);
+    Row C = finish<Stage>(d, L, rC, j, dt, inv_dt);
+    Row D = finish<Stage>(d, L, rD, j + step, dt, inv_dt);
+    FaceStates yC;
+    Flux gB;          // flux of the face between B and C
+    float byC = rC.by, byD = rD.by, byB = rB.by;
+    if constexpr (!Up) {
+        const FaceStates yB = construct<Textbook>(A.h, B.h, C.h, A.b, B.b, C.b, A.hu, B.hu, C.hu,
+                                                  A.hv, B.hv, C.hv, byC, rB.by);
+        yC = construct<Textbook>(B.h, C.h, D.h, B.b, C.b, D.b, B.hu, C.hu, D.hu, B.hv, C.hv, D.hv,
+                                 byD, byC);
+        gB = face_flux<1>(yB.hp, yB.qxp, yB.qyp, yC.hm, yC.qxm, yC.qym, kappa);
+    } else {
+        // rows: A = j+2, B = j+1 (north of C), D = j-1
+        const float byA = ld(d.BY + row_of(d, j + 2), L.c4);
+        const FaceStates yB = construct<Textbook>(C.h, B.h, A.h, C.b, B.b, A.b, C.hu, B.hu, A.hu,
+                                                  C.hv, B.hv, A.hv, byA, byB);
+        yC = construct<Textbook>(D.h, C.h, B.h, D.b, C.b, B.b, D.hu, C.hu, B.hu, D.hv, C.hv, B.hv,
+                                 byB, byC);
+        gB = face_flux<1>(yC.hp, yC.qxp, yC.qyp, yB.hm, yB.qxm, yB.qym, kappa);
+    }
+
+    float mxl = 0.f, myl = 0.f;
+    bool wetl = false;
+    for (; j != jstop; j += step) {
+        const int rj = (j - 2) / kTileH;   // uniform
+        if (rj != rr) {
+            flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane);
+            mxl = 0.f;
+            myl = 0.f;
+            wetl = false;
+            shift_lane<Up>(d, L, rj);
+            rr = rj;
+        }
+        // prefetch: the row after next and the next output row's inputs
+        // (clamped to the domain; unused past the end of the run)
+        const int jn = Up ? max(j - 3, 0) : j + 3;
+        const Raw rN = fetch<Stage>(d, L, jn);
+        const Src srcN = fetch_src(d, L, min(max(j + step, 2), d.ny - 3));
+
+        const Row E = finish<Stage>(d, L, rE, j + 2 * step, dt, inv_dt);
+        const float byE = rE.by;
+        FaceStates yD;
+        Flux gD;      // flux of the face between C and D
+        if constexpr (!Up) {
+            yD = construct<Textbook>(C.h, D.h, E.h, C.b, D.b, E.b, C.hu, D.hu, E.hu, C.hv, D.hv, E.hv,
+                                     byE, byD);
+            gD = face_flux<1>(yC.hp, yC.qxp, yC.qyp, yD.hm, yD.qxm, yD.qym, kappa);
+        } else {
+            yD = construct<Textbook>(E.h, D.h, C.h, E.b, D.b, C.b, E.hu, D.hu, C.hu, E.hv, D.hv, C.hv,
+                                     byC, byD);
+            gD = face_flux<1>(yD.hp, yD.qxp, yD.qyp, yC.hm, yC.qxm, yC.qym, kappa);
+        }
+        // north / south faces of row j
+        const Flux& gn = Up ? gB : gD;
+        const Flux& gs = Up ? gD : gB;
+
+        // x pass of row j
+        const float bxw = src.bx;
+        const float bxe = from_right(bxw);
+        const FaceStates xs = construct<Textbook>(
+            from_left(C.h), C.h, from_right(C.h), from_left(C.b), C.b, from_right(C.b),
+            from_left(C.hu), C.hu, from_right(C.hu), from_left(C.hv), C.hv, from_right(C.hv),
+            bxe, bxw);
+        const Flux fE = face_flux<0>(xs.hp, xs.qxp, xs.qyp, from_right(xs.hm), from_right(xs.qxm),
+                                     from_right(xs.qym), kappa);
+        const float fw0 = from_left(fE.f0), fw1 = from_left(fE.f1), fw2 = from_left(fE.f2);
+
+        if (out_lane && L.act1) {
+            const float dh = flux_divergence(fE.f0, fw0, gn.f0, gs.f0, S.inv_dx, S.inv_dy);
+            const float dhu = flux_divergence(fE.f1, fw1, gn.f1, gs.f1, S.inv_dx, S.inv_dy);
+            const float dhv = flux_divergence(fE.f2, fw2, gn.f2, gs.f2, S.inv_dx, S.inv_dy);
+            mxl = fmaxf(mxl, fE.speed);
+            myl = fmaxf(myl, gn.speed);
+            // beds of the south / north faces of row j
+            const float bys = byC, byn = Up ? byB : byD;
+            wetl |= output_row<Stage>(d, S, L, c, j, C, src, dh, dhu, dhv, bxw, bxe, bys, byn);
+        }
+        // roll the window
+        if constexpr (Up) byB = byC;
+        C = D;
+        D = E;
+        byC = byD;
+        byD = byE;
+        yC = yD;
+        gB = gD;
+        rE = rN;
+        src = srcN;
+    }
+    flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane);
+}
+
+// Work partition.  The launch holds exactly the waves that fit on the
+// device at once, so that every wave runs in a single round: units handed
+// out one per wave leave a second, nearly empty round whose lone waves are
+// latency-bound.  Workgroup b runs on XCD b % 8 (each with its own L2); with
+// Dev::band the waves are dealt so that XCD x holds the x-th eighth of the
+// work order below.
+//
+// Every unit active (dense): each strip is cut into m equal row shares, one
+// per wave, ordered share-major, so the waves marching the same rows of
+// neighbouring strips are neighbours in the order, run in step and share
+// their halo columns in L2.  Even shares march down, odd shares up: two
+// shares meet at a boundary either both at their first rows or both at
+// their last, so the halo rows around it are read once from HBM.
+//
+// Otherwise: the active units (strip-major) form a sequence of rows and wave
+// i marches the i-th equal share of at least Dev::min_rows rows, downwards,
+// continuing from one unit into the next while they are vertically adjacent.
 template <int Stage, bool Textbook, int Waves>
 __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
 {
@@ -273,227 +563,76 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
     // every unit and row index below is scalar too (SALU address math,
     // scalar branches)
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wave = blockIdx.x * 4 + wid;
-    const int nwaves = gridDim.x * 4;
     // stage 1 runs before the dt kernel moves the unit count
     const int nunits = min((Stage == 1) ? d.sc->n_units_acc : d.sc->n_units, d.nunits);
-    const float dt = (float)d.sc->dt, inv_dt = 1.f / dt;
-    const float dtp = (float)d.sc->dt_prev, inv_dtp = 1.f / dtp;
-    const float kappa = d.kappa;
-    const unsigned fl = d.flags;
-    const float rain_rate = d.sc->rain_rate, drain_rate = d.sc->drain_rate;
-    const float src_rate = d.sc->src_cell_rate;
-    const float tnow = (float)d.sc->t;
-    const float inv_dx = d.inv_dx, inv_dy = d.inv_dy;
+    if (nunits <= 0) return;
 
-    // Work order.  Workgroup b runs on XCD b % 8, each XCD with its own L2.
-    // When every unit is active, XCD x takes the x-th eighth of the units
-    // in row-major order, so the units that share halo rows and boundary
-    // cache lines with a unit (its vertical and horizontal neighbours) run
-    // on the same XCD at about the same time.  Otherwise the compacted list
-    // is walked grid-stride.
-    const bool dense = d.band && nunits == d.nunits;
-    int kbeg, kend, kstep;
-    if (dense) {
-        // bands = min(8, workgroups): every band has at least one workgroup
-        const int nb = min((int)gridDim.x, 8);
-        const int x = (int)blockIdx.x % nb;
-        const int nwg_x = ((int)gridDim.x - x + nb - 1) / nb;
-        kbeg = (int)((long long)d.nunits * x / nb) + ((int)blockIdx.x / nb) * 4 + wid;
-        kend = (int)((long long)d.nunits * (x + 1) / nb);
-        kstep = nwg_x * 4;
-    } else {
-        kbeg = wave;
-        kend = nunits;
-        kstep = nwaves;
+    const int G = (int)gridDim.x;
+    int pos = (int)blockIdx.x;
+    if (d.band) {
+        const int nb = min(G, 8), x = (int)blockIdx.x % nb, q = G / nb, r = G % nb;
+        pos = x * q + min(x, r) + (int)blockIdx.x / nb;
     }
-    (void)nwaves;
-    for (int k = kbeg; k < kend; k += kstep) {
-        const int u = __builtin_amdgcn_readfirstlane(dense ? k : d.ulist[k]);
-        const int s = u % d.nstrips, rr = u / d.nstrips;
-        Lane L;
-        const int c = kStrip * s + lane;
-        const int sc = min(max(c, 2), d.nx - 3);
-        L.c4 = 4u * (unsigned)c;
-        L.sc4 = 4u * (unsigned)sc;
-        L.pl4 = 4u * (unsigned)d.soff;
-        L.gx = (c < 2) || (c > d.nx - 3);
-        L.txo = (sc - 2) / kTileW;
-        L.r = rr;
-        {
-            const int Tm = max(rr - 1, 0) * d.nbx + L.txo, T0 = rr * d.nbx + L.txo;
-            const int Tp = min(rr + 1, d.nby - 1) * d.nbx + L.txo;
-            L.sel0 = d.sel[Tm] != 0; L.sel1 = d.sel[T0] != 0; L.sel2 = d.sel[Tp] != 0;
-            L.act0 = d.act[Tm] != 0; L.act1 = d.act[T0] != 0; L.act2 = d.act[Tp] != 0;
+    const int W = 4 * G;
+    const int wp = 4 * pos + wid;
+
+    StepScalars S;
+    S.dt = (float)d.sc->dt;
+    S.inv_dt = 1.f / S.dt;
+    S.dtp = (float)d.sc->dt_prev;
+    S.inv_dtp = 1.f / S.dtp;
+    S.kappa = d.kappa;
+    S.fl = d.flags;
+    S.rain_rate = d.sc->rain_rate;
+    S.drain_rate = d.sc->drain_rate;
+    S.src_rate = d.sc->src_cell_rate;
+    S.tnow = (float)d.sc->t;
+    S.inv_dx = d.inv_dx;
+    S.inv_dy = d.inv_dy;
+
+    const int rows = d.ny - 4;   // interior rows of a strip
+    const int mlim = max(1, rows / max(d.min_rows, 1));
+    const int m = min(W / d.nstrips, mlim);
+    if (nunits == d.nunits && m >= 2) {
+        const int mi = wp / d.nstrips, s = wp - mi * d.nstrips;
+        if (mi >= m) return;
+        const int ja = 2 + (int)((long long)rows * mi / m);
+        const int jb = 2 + (int)((long long)rows * (mi + 1) / m);
+        if (ja >= jb) return;
+        if (mi & 1) march_run<Stage, Textbook, true>(d, S, s, ja, jb, lane);
+        else march_run<Stage, Textbook, false>(d, S, s, ja, jb, lane);
+        return;
+    }
+
+    const long long total = (long long)nunits * kTileH;
+    const long long nshare = max(1LL, min((long long)W, total / max(d.min_rows, 1)));
+    // shares [i0, i1) of this wave (at most one when nshare <= W)
+    const long long i0 = ((long long)wp * nshare + W - 1) / W;
+    const long long i1 = ((long long)(wp + 1) * nshare + W - 1) / W;
+    if (i0 >= i1) return;
+    const long long p0 = total * i0 / nshare, pend = total * i1 / nshare;
+    if (p0 >= pend) return;
+    int k = (int)(p0 / kTileH);
+    int a = (int)(p0 - (long long)k * kTileH);
+    for (;;) {
+        // a maximal run of vertically adjacent units of one strip
+        const int u = __builtin_amdgcn_readfirstlane(nunits == d.nunits ? k : d.ulist[k]);
+        const int s = u / d.nurows, rr = u - s * d.nurows;
+        const int ja = rr * kTileH + 2 + a;
+        int last = u, kk = k;
+        while ((long long)(kk + 1) * kTileH < pend) {
+            const int un = __builtin_amdgcn_readfirstlane(nunits == d.nunits ? kk + 1 : d.ulist[kk + 1]);
+            if (un != last + 1 || un - s * d.nurows <= 0 || un - s * d.nurows >= d.nurows) break;
+            last = un;
+            ++kk;
         }
-        const bool out_lane = lane >= 2 && lane <= kStrip + 1 && c <= d.nx - 3;
-        const bool act_own = L.act1;
-        const int j0 = rr * kTileH + 2;
-        const int jend = min(j0 + kTileH, d.ny - 2);
-
-        const Row A = finish<Stage>(d, L, fetch<Stage>(d, L, j0 - 2), j0 - 2, dt, inv_dt);
-        const Raw rB = fetch<Stage>(d, L, j0 - 1);
-        const Raw rC = fetch<Stage>(d, L, j0);
-        const Raw rD = fetch<Stage>(d, L, j0 + 1);
-        Raw rE = fetch<Stage>(d, L, j0 + 2);
-        Src src = fetch_src(d, L, j0);
-        const Row B = finish<Stage>(d, L, rB, j0 - 1, dt, inv_dt);
-        Row C = finish<Stage>(d, L, rC, j0, dt, inv_dt);
-        Row D = finish<Stage>(d, L, rD, j0 + 1, dt, inv_dt);
-        float byC = rC.by, byD = rD.by;
-        FaceStates yB = construct<Textbook>(A.h, B.h, C.h, A.b, B.b, C.b, A.hu, B.hu, C.hu,
-                                            A.hv, B.hv, C.hv, byC, rB.by);
-        FaceStates yC = construct<Textbook>(B.h, C.h, D.h, B.b, C.b, D.b, B.hu, C.hu, D.hu,
-                                            B.hv, C.hv, D.hv, byD, byC);
-        Flux gS = face_flux<1>(yB.hp, yB.qxp, yB.qyp, yC.hm, yC.qxm, yC.qym, kappa);
-
-        float mxl = 0.f, myl = 0.f;
-        bool wetl = false;
-        for (int j = j0; j < jend; ++j) {
-            // prefetch: the row after next and the next output row's inputs
-            const Raw rN = fetch<Stage>(d, L, min(j + 3, jend + 1));
-            const Src srcN = fetch_src(d, L, min(j + 1, jend - 1));
-
-            const Row E = finish<Stage>(d, L, rE, j + 2, dt, inv_dt);
-            const float byE = rE.by;
-            const FaceStates yD = construct<Textbook>(C.h, D.h, E.h, C.b, D.b, E.b, C.hu, D.hu,
-                                                      E.hu, C.hv, D.hv, E.hv, byE, byD);
-            const Flux gN = face_flux<1>(yC.hp, yC.qxp, yC.qyp, yD.hm, yD.qxm, yD.qym, kappa);
-
-            // x pass of row j
-            const float bxw = src.bx;
-            const float bxe = from_right(bxw);
-            const FaceStates xs = construct<Textbook>(
-                from_left(C.h), C.h, from_right(C.h), from_left(C.b), C.b, from_right(C.b),
-                from_left(C.hu), C.hu, from_right(C.hu), from_left(C.hv), C.hv, from_right(C.hv),
-                bxe, bxw);
-            const Flux fE = face_flux<0>(xs.hp, xs.qxp, xs.qyp, from_right(xs.hm),
-                                         from_right(xs.qxm), from_right(xs.qym), kappa);
-            const float fw0 = from_left(fE.f0), fw1 = from_left(fE.f1), fw2 = from_left(fE.f2);
-
-            if (out_lane && act_own) {
-                const size_t rbo = row_of(d, j);   // output row (uniform)
-                float dh = flux_divergence(fE.f0, fw0, gN.f0, gS.f0, inv_dx, inv_dy);
-                float dhu = flux_divergence(fE.f1, fw1, gN.f1, gS.f1, inv_dx, inv_dy);
-                float dhv = flux_divergence(fE.f2, fw2, gN.f2, gS.f2, inv_dx, inv_dy);
-                mxl = fmaxf(mxl, fE.speed);
-                myl = fmaxf(myl, gN.speed);
-
-                const float hC = C.h;
-                const float S1 = -kG * hC * ((bxe - bxw) * inv_dx);
-                const float S2 = -kG * hC * ((byD - byC) * inv_dy);
-                float S0 = 0.f;
-                if (fl & RIFT_RAIN_AVG) S0 += rain_rate;
-                if (fl & RIFT_RAIN_GRID) S0 += src.rain;
-                if (fl & RIFT_DRAIN_AVG) {
-                    const float cap = hC * inv_dt;
-                    S0 -= (drain_rate > cap) ? cap : drain_rate;
-                }
-                if (fl & RIFT_DRAIN_GRID) {
-                    const float cap = hC * inv_dt;
-                    S0 -= (src.drain > cap) ? cap : src.drain;
-                }
-                const bool srcc = (fl & RIFT_HYDROGRAPH) && c >= d.si0 && c <= d.si1 &&
-                                  j >= d.sj0 && j <= d.sj1;
-                if (srcc) S0 += src_rate;
-                float Fnew = 0.f;
-                if (fl & RIFT_INFILTRATION) {
-                    if constexpr (Stage == 1) {
-                        S0 -= green_ampt(src.K, src.F, d.psi_dtheta, hC, dt, inv_dt);
-                    } else {
-                        // the stage-1 rate (previous dt, stored depth), then
-                        // F* = F + dF dt of Integrate_1_k (grid.cu:917-923)
-                        const float f1 = green_ampt(src.K, src.F, d.psi_dtheta, C.hraw, dtp, inv_dtp);
-                        const float Fs = fmaf(f1, dt, src.F);
-                        const float f2 = green_ampt(src.K, Fs, d.psi_dtheta, hC, dt, inv_dt);
-                        S0 -= f2;
-                        Fnew = 0.5f * fmaf(f2, dt, src.F + Fs);
-                    }
-                }
-                dh += S0;
-                dhu += S1;
-                dhv += S2;
-                if constexpr (Stage == 1) {
-                    at(d.dh + rbo, L.c4) = dh;
-                    at(d.dhu + rbo, L.c4) = dhu;
-                    at(d.dhv + rbo, L.c4) = dhv;
-                } else {
-                    // Integrate_2_k (grid.cu:1077-1123)
-                    // U^n depth with the dry clamp of ComputeFluxes_k
-                    const float h_n = (C.hraw > kEps) ? C.hraw : 0.f;
-                    const float hn = 0.5f * fmaf(dh, dt, h_n + C.h);
-                    const float hun = 0.5f * fmaf(dhu, dt, C.hun + C.hu);
-                    const float hvn = 0.5f * fmaf(dhv, dt, C.hvn + C.hv);
-                    const float hc = (hn > kEps) ? hn : 0.f;
-                    // into the buffer the tile does not read
-                    const unsigned lo = L.c4 + (L.sel1 ? 0u : L.pl4);
-                    at(d.h + rbo, lo) = hc;
-                    at(d.hu + rbo, lo) = hun;
-                    at(d.hv + rbo, lo) = hvn;
-                    if (fl & RIFT_INFILTRATION) at(d.F + rbo, L.c4) = Fnew;
-                    if (fl & RIFT_SAVE_MAX) {
-                        const float fv = vel_factor(hc, kappa);
-                        const float uC = fv * hun, vC = fv * hvn;
-                        float& hm = at(d.hmax + rbo, L.c4);
-                        float& qm = at(d.qmax + rbo, L.c4);
-                        hm = fmaxf(hm, hc);
-                        qm = fmaxf(qm, hc * fsqrt(vC * vC + uC * uC));
-                    }
-                    if (fl & RIFT_SAVE_ARRIVAL) {
-                        float& tw = at(d.twet + rbo, L.c4);
-                        if (tw < 0.f && hc > kEps) tw = tnow;
-                    }
-                    bool w = hc > kEps || srcc;
-                    if ((fl & RIFT_RAIN_AVG) && rain_rate > 0.f) w = true;
-                    if ((fl & RIFT_RAIN_GRID) && src.rain > 0.f) w = true;
-                    wetl |= w;
-                }
-            }
-            // roll the window
-            C = D;
-            D = E;
-            byC = byD;
-            byD = byE;
-            yC = yD;
-            gS = gN;
-            rE = rN;
-            src = srcN;
-        }
-
-        // per-tile speed maxima: stage 1 into mx1/my1 (this step's dt),
-        // stage 2 into mx/my (what later steps see while the tile is
-        // inactive); stage 2 also forms the tile wet flags
-        float* const tmx = (Stage == 1) ? d.mx1 : d.mx;
-        float* const tmy = (Stage == 1) ? d.my1 : d.my;
-        int txa, txb;
-        strip_tiles(d, s, txa, txb);
-        for (int tx = txa; tx <= txb; ++tx) {
-            const int T = rr * d.nbx + tx;
-            if (!d.act[T]) continue;   // uniform
-            const bool seg = out_lane && L.txo == tx;
-            const float m1 = wave_max(seg ? mxl : 0.f);
-            const float m2 = wave_max(seg ? myl : 0.f);
-            const bool anyw = __ballot(seg && wetl) != 0ull;
-            if (lane == 0) {
-                atomicMax(reinterpret_cast<unsigned*>(tmx) + T, __float_as_uint(m1));
-                atomicMax(reinterpret_cast<unsigned*>(tmy) + T, __float_as_uint(m2));
-                if constexpr (Stage == 2) {
-                    d.flip[T] = 1;
-                    // one atomic per tile and unit: count down, count wet
-                    // contributors; the last contributor sees both totals
-                    const unsigned old = atomicAdd(d.pending + T, (anyw ? 1u : 0u) - (1u << 8));
-                    if ((old >> 8) == 1u) {
-                        // Integrate_2_k's flag write, by thread (0,0) of the
-                        // tile, i.e. only when the tile's origin cell is
-                        // interior (grid.cu:1160-1170)
-                        const bool acc = ((old & 0xFFu) + (anyw ? 1u : 0u)) != 0u;
-                        const int bx = T % d.nbx, by = T / d.nbx;
-                        if (bx * kTileW + 2 <= d.nx - 3 && by * kTileH + 2 <= d.ny - 3)
-                            d.wet[T] = (!(fl & RIFT_TILE_ACCEL) || acc) ? 1 : 0;
-                    }
-                }
-            }
-        }
+        const int rlast = last - s * d.nurows;
+        const long long pe = min(pend, (long long)(kk + 1) * kTileH);
+        const int jb = min(rlast * kTileH + 2 + (int)(pe - (long long)kk * kTileH), d.ny - 2);
+        if (ja < jb) march_run<Stage, Textbook, false>(d, S, s, ja, jb, lane);
+        k = kk + 1;
+        a = 0;
+        if ((long long)k * kTileH >= pend) break;
     }
 }
 
@@ -633,8 +772,9 @@ void launch_grow2(const Dev& d, hipStream_t s)
 
 // Launch variants (all give identical results; they differ in register
 // budget and work order): bit 0 = stage 1 at 4 instead of 5 waves per SIMD,
-// bit 1 = stage 2 at 3 instead of 4 waves per SIMD; the XCD banding of the
-// work order is Dev::band.
+// bit 1 = stage 2 at 3 instead of 4 waves per SIMD; the dealing of work
+// shares and their least size are Dev::band and Dev::min_rows.  The grid is
+// the number of workgroups resident at once (march_resident_blocks).
 template <int Stage, int Waves>
 static void launch_march_w(const Dev& d, int blocks, hipStream_t s)
 {
@@ -644,11 +784,9 @@ static void launch_march_w(const Dev& d, int blocks, hipStream_t s)
         hipLaunchKernelGGL((k_march<Stage, false, Waves>), dim3(blocks), dim3(256), 0, s, d);
 }
 
-void launch_march(const Dev& d, int stage, hipStream_t s, int variant)
+void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks)
 {
-    int blocks = (d.nunits + 3) / 4;
-    if (blocks > 2048) blocks = 2048;
-    if (blocks < 1) blocks = 1;
+    blocks = std::max(1, blocks);
     if (stage == 1) {
         if (variant & 1) launch_march_w<1, 4>(d, blocks, s);
         else launch_march_w<1, 5>(d, blocks, s);
@@ -656,6 +794,23 @@ void launch_march(const Dev& d, int stage, hipStream_t s, int variant)
         if (variant & 2) launch_march_w<2, 3>(d, blocks, s);
         else launch_march_w<2, 4>(d, blocks, s);
     }
+}
+
+template <int Stage, int Waves>
+static int resident(bool textbook, int cus)
+{
+    int per_cu = 0;
+    hipError_t e = textbook
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_march<Stage, true, Waves>, 256, 0)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_march<Stage, false, Waves>, 256, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    return per_cu * cus;
+}
+
+int march_resident_blocks(int stage, int variant, bool textbook, int cus)
+{
+    if (stage == 1) return (variant & 1) ? resident<1, 4>(textbook, cus) : resident<1, 5>(textbook, cus);
+    return (variant & 2) ? resident<2, 3>(textbook, cus) : resident<2, 4>(textbook, cus);
 }
 
 void launch_canonicalize(const Dev& d, hipStream_t s)

@@ -68,6 +68,8 @@ struct rift_grid {
     // counters clears it
     bool primed = false;
     int variant = 0;              // launch variant of the marching kernels
+    int cus = 0;                  // compute units of the device
+    int march_blocks[2][kMarchVariants] = {};   // resident workgroups per stage/variant
     bool per_phase = false;       // rift_step uses the one-kernel-per-phase path
     // captured fused steps (1 and kGraphSteps steps); the kernels take the
     // handle's Dev by value, so any change of it invalidates the graphs
@@ -228,6 +230,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.si1 = d.sj1 = 0;
     d.nsrc = 0;
     d.band = 1;
+    d.min_rows = kTileH;
 
     hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -271,18 +274,26 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.my1 = alloc_bytes<float>(g, d.ntiles);
     d.sel = alloc_bytes<unsigned char>(g, d.ntiles);
     d.flip = alloc_bytes<unsigned char>(g, d.ntiles);
-    d.pending = alloc_bytes<unsigned>(g, d.ntiles);
     d.ulist = alloc_bytes<int>(g, d.nunits);
     d.sc = alloc_bytes<Scalars>(g, 1);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
     ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.ser && g->dsum && d.sel &&
-         d.flip && d.pending && d.ulist;
+         d.flip && d.ulist;
     if (!ok) {
         rift_destroy(g);
         return fail(RIFT_E_HIP, "rift_create: device allocation failed");
     }
     g->grid_tiles = std::min(d.ntiles, 256 * 8);
+    {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, desc->device));
+        g->cus = prop.multiProcessorCount;
+        for (int st = 0; st < 2; ++st)
+            for (int v = 0; v < kMarchVariants; ++v)
+                g->march_blocks[st][v] =
+                    march_resident_blocks(st + 1, v, (desc->flags & RIFT_TEXTBOOK) != 0, g->cus);
+    }
     *out = g;
     return RIFT_OK;
 }
@@ -384,7 +395,6 @@ int rift_init(rift_grid* g, double t0)
     HIPCHK(hipMemsetAsync(d.my, 0, d.ntiles * sizeof(float), g->stream));
     HIPCHK(hipMemsetAsync(d.sel, 0, d.ntiles, g->stream));
     HIPCHK(hipMemsetAsync(d.flip, 0, d.ntiles, g->stream));
-    HIPCHK(hipMemsetAsync(d.pending, 0, d.ntiles * sizeof(unsigned), g->stream));
     g->scattered = false;
     Scalars sc{};
     HIPCHK(hipMemcpyAsync(&sc, d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, g->stream));
@@ -618,7 +628,7 @@ static int step_fused(rift_grid* g)
     }
     {
         ProfScope ps(g, 0);
-        launch_march(d, 1, g->stream, g->variant);
+        launch_march(d, 1, g->stream, g->variant, g->march_blocks[0][g->variant]);
     }
     {
         ProfScope ps(g, 5);
@@ -626,7 +636,7 @@ static int step_fused(rift_grid* g)
     }
     {
         ProfScope ps(g, 1);
-        launch_march(d, 2, g->stream, g->variant);
+        launch_march(d, 2, g->stream, g->variant, g->march_blocks[1][g->variant]);
     }
     g->scattered = true;
     return check_launch("fused step");
@@ -696,6 +706,7 @@ int rift_set_option(rift_grid* g, int option, int value)
         drop_graphs(g);   // the graphs hold the launch configuration
         g->variant = value & (kMarchVariants - 1);
         g->d.band = (value & kMarchVariants) ? 0 : 1;
+        g->d.min_rows = (value & (2 * kMarchVariants)) ? kTileH / 2 : kTileH;
         return RIFT_OK;
     default:
         return fail(RIFT_E_ARG, "rift_set_option: unknown option");

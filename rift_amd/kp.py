@@ -36,7 +36,7 @@ FIELDS = ["w", "hu", "hv", "depth", "bc", "n", "K", "F", "dw", "dhu", "dhv", "G"
 SER_HYETO, SER_DRAIN, SER_HYDRO = 0, 1, 2
 MAP_RAIN, MAP_DRAIN, MAP_SURGE = 0, 1, 2
 OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT = 1, 2, 3
-NVARIANTS = 8
+NVARIANTS = 16
 KERNEL_CLASSES = ["flux1", "stage2_flux", "integrate", "boundaries", "grow", "timestep"]
 
 # (name, restype, argtypes) of every exported entry point
