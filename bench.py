@@ -41,6 +41,23 @@ KERNEL_BYTES_PER_CELL = {"march1": 44, "march2": 60}
 STEP_BYTES_PER_WCU = 96
 
 
+PROFILE = os.path.join(ROOT, "profiles", "fused_step_rain_basin_2048.md")
+
+
+def measured_traffic(kernel, n_cells):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (calibrated against a copy of known size) for the 2048 x 2048
+    workload, as recorded in the committed profile summary; None for other
+    workloads or when the summary is absent."""
+    if n_cells != 2048 or not os.path.exists(PROFILE):
+        return None
+    key = f"traffic_bytes_per_launch {kernel}:"
+    for line in open(PROFILE):
+        if line.startswith(key):
+            return float(line[len(key):].strip())
+    return None
+
+
 def build_workload(n_cells):
     from rift_amd import kp, synth
     cfg = synth.rain_basin(n_cells, seed=2)
@@ -210,7 +227,10 @@ def main():
                    "active_tiles": int(n_act), "wet_cells_per_step": wcu / args.steps,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic("k_march<2>" if dom == "march2" else "k_march<1>",
+                                                 args.cells),
+                     "traffic_source": os.path.relpath(PROFILE, ROOT),
                      "kernel": "k_march<2>" if dom == "march2" else "k_march<1>",
                      "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "avg_launch_ms": kern[dom]["avg_launch_ms"]},
