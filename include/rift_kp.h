@@ -47,6 +47,7 @@ extern "C" {
 #define RIFT_TILE_ACCEL (1u << 10)   /* tile_acceleration                 */
 #define RIFT_H_INIT (1u << 11)       /* h_init                            */
 #define RIFT_RAIN_INTERP (1u << 12)  /* hyetograph_interp (host series)   */
+#define RIFT_MASS_BUDGET (1u << 13)  /* check_volume: FP64 volume budget   */
 #define RIFT_TEXTBOOK (1u << 16)     /* extension: sign-aware minmod      */
 
 /* Fields for rift_download / rift_upload (nx*ny doubles). */
@@ -219,6 +220,17 @@ int rift_sum_interior(rift_grid* g, int field, double* out);
  * out[2] = rain volume added, out[3] = source volume added,
  * out[4] = drain volume removed, out[5] = boundary outflow volume. */
 int rift_mass_balance(rift_grid* g, double* out6);
+/* FP64 volume budget (m^3) since the last rift_budget_reset (rift_init
+ * resets), accumulated on the device by the fused step when the handle was
+ * created with RIFT_MASS_BUDGET (no reference counterpart: the reference
+ * prints a host-side sum, simulator.cu:609-637, 820-842):
+ * out[0] stored water, out[1] infiltrated, out[2] stored + infiltrated at the
+ * reset, out[3] rain in, out[4] source in, out[5] drain out, out[6] outflow
+ * through the domain boundary, out[7] exchange of the dry clamps and the
+ * surge override, out[8] residual = change of stored + infiltrated minus
+ * (rain + source - drain - outflow + exchange). */
+int rift_budget(rift_grid* g, double* out9);
+int rift_budget_reset(rift_grid* g);
 
 /* The handle's HIP stream (hipStream_t), for callers that interleave work. */
 int rift_get_stream(rift_grid* g, void** stream);

@@ -290,6 +290,15 @@ struct orc_sim {
     double* src_rate;
     map_series ms[3];
     double v_rain, v_src;
+    /* FP64 volume budget since orc_budget_reset (m^3): stored + infiltrated
+     * volume at the reset, then rain and source in, drain and outflow out
+     * (through the faces of the active region: the domain boundary and faces
+     * to inactive tiles), and the exchange of the dry clamps and the surge
+     * override.  Each flux pass leaves its per-cell rate sums (m/s summed
+     * over cells) and its clamp change (m summed over cells) in st_*; the
+     * integration stage that follows weights them with its share of dt. */
+    double b_v0, b_rain, b_src, b_drain, b_out, b_exch;
+    double st_rain, st_src, st_drain, st_out, st_clamp;
 };
 
 #define IX(s, i, j) ((size_t)(j) * (size_t)(s)->nx + (size_t)(i))
@@ -668,7 +677,18 @@ void orc_grow(orc_sim* s)
 
 /* --- ComputeFluxes_k, grid.cu:564-839 ------------------------------------- */
 
-static void flux_tile(orc_sim* s, int tile, double dtd)
+typedef struct stage_sums {
+    double rain, src, drain, out;
+} stage_sums;
+
+/* active_blocks value >= 0 marks an active tile (tile 0 always, -0 == 0) */
+static int tile_active(const orc_sim* s, int bx, int by)
+{
+    if (bx < 0 || by < 0 || bx >= s->nbx || by >= s->nby) return 0;
+    return s->aval[by * s->nbx + bx] >= 0;
+}
+
+static void flux_tile(orc_sim* s, int tile, double dtd, stage_sums* sums)
 {
     const int m = s->mode, nx = s->nx, ny = s->ny;
     const real kappa = (real)s->kappa, dx = (real)s->dx, dy = (real)s->dy;
@@ -698,6 +718,9 @@ static void flux_tile(orc_sim* s, int tile, double dtd)
             sp[f - i0] = face_r(m, 0, L[0], L[1], L[2], R[3], R[4], R[5],
                                 s->BX[IBX(s, f, j)], kappa, Hf[f - i0]);
         }
+        /* the tile's west / east faces on the edge of the active region */
+        if (i0 == 2 || !tile_active(s, bx - 1, by)) sums->out -= (double)(Hf[0][0] / dx);
+        if (i1 == nx - 2 || !tile_active(s, bx + 1, by)) sums->out += (double)(Hf[i1 - i0][0] / dx);
         for (int i = i0; i < i1; ++i) {
             size_t q = IX(s, i, j);
             s->dw[q] = -(Hf[i + 1 - i0][0] - Hf[i - i0][0]) / dx;
@@ -722,6 +745,8 @@ static void flux_tile(orc_sim* s, int tile, double dtd)
             spy[f - j0] = face_r(m, 1, L[0], L[1], L[2], R[3], R[4], R[5],
                                  s->BY[IBY(s, i, f)], kappa, Hg[f - j0]);
         }
+        if (j0 == 2 || !tile_active(s, bx, by - 1)) sums->out -= (double)(Hg[0][0] / dy);
+        if (j1 == ny - 2 || !tile_active(s, bx, by + 1)) sums->out += (double)(Hg[j1 - j0][0] / dy);
         for (int j = j0; j < j1; ++j) {
             size_t q = IX(s, i, j);
             s->dw[q] += -(Hg[j + 1 - j0][0] - Hg[j - j0][0]) / dy;
@@ -754,18 +779,26 @@ static void flux_tile(orc_sim* s, int tile, double dtd)
             real S0 = 0;
             if (s->flags & ORC_RAIN_AVG) S0 += (real)hy->interpolated;
             if (s->flags & ORC_RAIN_GRID) S0 += s->ms[ORC_MAP_RAIN].cur[q];
+            sums->rain += (double)S0;
             if (s->flags & ORC_DRAIN_AVG) {
                 real d = (real)dr->interpolated;
-                S0 -= (d > hC / dt) ? hC / dt : d;
+                real taken = (d > hC / dt) ? hC / dt : d;
+                S0 -= taken;
+                sums->drain += (double)taken;
             }
             if (s->flags & ORC_DRAIN_GRID) {
                 real d = s->ms[ORC_MAP_DRAIN].cur[q];
-                S0 -= (d > hC / dt) ? hC / dt : d;
+                real taken = (d > hC / dt) ? hC / dt : d;
+                S0 -= taken;
+                sums->drain += (double)taken;
             }
             if ((s->flags & ORC_HYDROGRAPH) || s->nsrc > 0) {
                 long cell = (long)j * nx + i;
                 for (long k = 0; k < s->nsrc; ++k)
-                    S0 = (cell == s->src_idx[k]) ? S0 + (real)s->src_rate[k] : S0;
+                    if (cell == s->src_idx[k]) {
+                        S0 += (real)s->src_rate[k];
+                        sums->src += s->src_rate[k];
+                    }
             }
             if (s->flags & ORC_INFILTRATION) {
                 real Kq = s->K[q], Fq = s->F[q];
@@ -790,12 +823,20 @@ void orc_compute_fluxes(orc_sim* s)
 {
     const int nl = s->nlist;
     const double dt = s->dt;
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int k = 0; k < nl; ++k) flux_tile(s, s->list[k], dt);
+    double r_rain = 0, r_src = 0, r_drain = 0, r_out = 0, r_clamp = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : r_rain, r_src, r_drain, r_out)
+    for (int k = 0; k < nl; ++k) {
+        stage_sums sm = {0, 0, 0, 0};
+        flux_tile(s, s->list[k], dt, &sm);
+        r_rain += sm.rain;
+        r_src += sm.src;
+        r_drain += sm.drain;
+        r_out += sm.out;
+    }
     /* The dry clamp w <- BC (grid.cu:748-749) is written after every tile has
      * read its stencil: the reference's cross-tile halo race is resolved as
      * "reads see the values from before the kernel". */
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : r_clamp)
     for (int k = 0; k < nl; ++k) {
         int tile = s->list[k];
         int bx = tile % s->nbx, by = tile / s->nbx;
@@ -804,10 +845,32 @@ void orc_compute_fluxes(orc_sim* s)
                 int i = bx * TILE_W + tx + 2, j = by * TILE_H + ty + 2;
                 if (i > s->nx - 3 || j > s->ny - 3) continue;
                 size_t q = IX(s, i, j);
-                if (s->clamp[q]) s->w[q] = s->BC[q];
+                if (s->clamp[q]) {
+                    r_clamp += (double)(s->BC[q] - s->w[q]);
+                    s->w[q] = s->BC[q];
+                }
             }
         }
     }
+    s->st_rain = r_rain;
+    s->st_src = r_src;
+    s->st_drain = r_drain;
+    s->st_out = r_out;
+    s->st_clamp = r_clamp;
+}
+
+/* Books the last flux pass into the budget with the weight its rates carry
+ * in the stage's update (dt for Euler's single stage, dt/2 for each Heun
+ * stage) and its clamp change with clamp_weight. */
+static void book_stage(orc_sim* s, double rate_weight, double clamp_weight, double exch)
+{
+    const double area = s->dx * s->dy;
+    s->b_rain += rate_weight * s->st_rain * area;
+    s->b_src += rate_weight * s->st_src * area;
+    s->b_drain += rate_weight * s->st_drain * area;
+    s->b_out += rate_weight * s->st_out * area;
+    s->b_exch += (clamp_weight * s->st_clamp + exch) * area;
+    s->st_rain = s->st_src = s->st_drain = s->st_out = s->st_clamp = 0;
 }
 
 /* ComputeTimestep, simulator.cu:540-566: max over every tile's (possibly
@@ -885,7 +948,8 @@ void orc_integrate_1(orc_sim* s)
     const real dt = (real)s->dt;
     const int euler = (s->flags & ORC_EULER) ? 1 : 0;
     const real nd = (real)s->nodata;
-#pragma omp parallel for schedule(dynamic, 4)
+    double exch = 0;   /* surge override and dry clamp, m summed over cells */
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : exch)
     for (int k = 0; k < s->nlist; ++k) {
         int tile = s->list[k];
         int bx = tile % s->nbx, by = tile / s->nbx, any = 0;
@@ -902,7 +966,9 @@ void orc_integrate_1(orc_sim* s)
                     if (sg != nd && sg > (real)0.0) {
                         real next_h = s->w[q] + s->dw[q] * dt - s->BC[q];
                         real deltaw = sg - next_h;
+                        real before = s->dw[q];
                         s->dw[q] = deltaw / dt;
+                        exch += (double)((s->dw[q] - before) * dt);
                     }
                 }
                 s->w[q] += s->dw[q] * dt;
@@ -911,6 +977,7 @@ void orc_integrate_1(orc_sim* s)
                 s->hv[q] += s->dhv[q] * dt;
                 s->hv[q] /= ((real)1.0f - dt * s->G[q]);
                 real hC = (s->w[q] - s->BC[q] > EPS) ? s->w[q] - s->BC[q] : (real)0.f;
+                if (!(hC > EPS)) exch += (double)(s->BC[q] - s->w[q]);
                 s->w[q] = (hC > EPS) ? s->w[q] : s->BC[q];
                 if (s->flags & ORC_INFILTRATION) {
                     s->F_old[q] = s->F[q];
@@ -924,13 +991,18 @@ void orc_integrate_1(orc_sim* s)
         }
         if (euler) tile_wet_flag(s, tile, any);
     }
+    /* the stage-1 flux pass's clamp persists in w_old (weight 1); the rates
+     * and this stage's exchange enter U^{n+1} with weight 1/2 (Heun) or 1 */
+    const double wgt = euler ? 1.0 : 0.5;
+    book_stage(s, wgt * s->dt, 1.0, wgt * exch);
 }
 
 /* Integrate_2_k, grid.cu:1043-1171 */
 void orc_integrate_2(orc_sim* s)
 {
     const real dt = (real)s->dt;
-#pragma omp parallel for schedule(dynamic, 4)
+    double exch = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : exch)
     for (int k = 0; k < s->nlist; ++k) {
         int tile = s->list[k];
         int bx = tile % s->nbx, by = tile / s->nbx, any = 0;
@@ -943,6 +1015,7 @@ void orc_integrate_2(orc_sim* s)
                 s->hu[q] = (real)0.5f * (s->hu_old[q] + s->hu[q] + s->dhu[q] * dt);
                 s->hv[q] = (real)0.5f * (s->hv_old[q] + s->hv[q] + s->dhv[q] * dt);
                 real hC = (s->w[q] - s->BC[q] > EPS) ? s->w[q] - s->BC[q] : (real)0.f;
+                if (!(hC > EPS)) exch += (double)(s->BC[q] - s->w[q]);
                 s->w[q] = (hC > EPS) ? s->w[q] : s->BC[q];
                 if (s->flags & ORC_INFILTRATION)
                     s->F[q] = (real)0.5f * (s->F_old[q] + s->F[q] + s->dF[q] * dt);
@@ -952,6 +1025,8 @@ void orc_integrate_2(orc_sim* s)
         }
         tile_wet_flag(s, tile, any);
     }
+    /* the stage-2 flux pass's clamp acts on U* (weight 1/2) */
+    book_stage(s, 0.5 * s->dt, 0.5, exch);
 }
 
 void orc_update_maps(orc_sim* s)
@@ -1072,6 +1147,7 @@ orc_sim* orc_create(const orc_params* p, const double* dem, const double* h0c,
     s->t = p->t0;
     s->dt = (double)0.0000001f;                          /* simulator.cu:35 */
     orc_apply_boundaries(s);                              /* simulator.cu:731 */
+    orc_budget_reset(s);
     return s;
 }
 
@@ -1178,6 +1254,40 @@ void orc_get_rates(const orc_sim* s, float* out)
     out[0] = s->ser[ORC_SER_HYETO].interpolated;
     out[1] = s->ser[ORC_SER_DRAIN].interpolated;
     out[2] = s->ser[ORC_SER_HYDRO].interpolated;
+}
+
+static double total_volume(const orc_sim* s)
+{
+    double v = 0.0;
+    for (int j = 2; j <= s->ny - 3; ++j)
+        for (int i = 2; i <= s->nx - 3; ++i) {
+            size_t q = IX(s, i, j);
+            v += (double)(s->w[q] - s->BC[q]);
+            if (s->flags & ORC_INFILTRATION) v += (double)s->F[q];
+        }
+    return v * s->dx * s->dy;
+}
+
+void orc_budget_reset(orc_sim* s)
+{
+    s->b_v0 = total_volume(s);
+    s->b_rain = s->b_src = s->b_drain = s->b_out = s->b_exch = 0;
+}
+
+void orc_budget(const orc_sim* s, double* out)
+{
+    double m[4];
+    orc_mass(s, m);
+    const double total = m[0] + ((s->flags & ORC_INFILTRATION) ? m[1] : 0.0);
+    out[0] = m[0];
+    out[1] = m[1];
+    out[2] = s->b_v0;
+    out[3] = s->b_rain;
+    out[4] = s->b_src;
+    out[5] = s->b_drain;
+    out[6] = s->b_out;
+    out[7] = s->b_exch;
+    out[8] = (total - s->b_v0) - (s->b_rain + s->b_src - s->b_drain - s->b_out + s->b_exch);
 }
 
 void orc_mass(const orc_sim* s, double* out)

@@ -146,6 +146,14 @@ void     orc_get_rates(const orc_sim* s, float* out);
 /* Mass balance: out[0] = sum interior depth * dx*dy, out[1] = sum F*dx*dy,
  * out[2] = cumulative rain volume added, out[3] = cumulative source volume. */
 void     orc_mass(const orc_sim* s, double* out);
+/* FP64 volume budget (m^3) since the last orc_budget_reset (orc_create
+ * resets): out[0] stored water, out[1] infiltrated, out[2] stored +
+ * infiltrated at the reset, out[3] rain in, out[4] source in, out[5] drain
+ * out, out[6] outflow through the faces of the active region, out[7]
+ * exchange of the dry clamps and the surge override, out[8] residual
+ * (change of stored + infiltrated minus the budget terms). */
+void     orc_budget(const orc_sim* s, double* out);
+void     orc_budget_reset(orc_sim* s);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,11 @@ struct Scalars {
     int next_valid;
     int n_active_acc;                 // dt kernel: cross-workgroup accumulators
     unsigned long long wet_cells_acc;
+    // volume budget of the fused step (m summed over cells): gridded rain,
+    // drain, outflow through the domain boundary, dry-clamp and surge
+    // exchange; b_flux1 = stage-1 boundary outflow rate (m/s summed over
+    // cells), weighted with dt/2 by the dt kernel
+    double b_rain_grid, b_drain, b_out, b_exch, b_flux1;
     float nxt_rain, nxt_drain, nxt_hydro, nxt_cell;
 };
 
@@ -178,7 +183,7 @@ void launch_grow2(const Dev& d, hipStream_t s);
 void launch_timestep2(const Dev& d, hipStream_t s);
 void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks);
 // workgroups of a marching kernel resident at once on a device of `cus` CUs
-int march_resident_blocks(int stage, int variant, bool textbook, int cus);
+int march_resident_blocks(int stage, int variant, bool textbook, bool budget, int cus);
 constexpr int kMarchVariants = 4;
 void launch_canonicalize(const Dev& d, hipStream_t s);
 

@@ -136,6 +136,7 @@ struct Row {
     float h, hu, hv, b;     // state used by the flux (U^n in stage 1, U* in 2)
     float hraw;             // stage 2: U^n depth as stored (stage-1 sources)
     float hun, hvn;         // stage 2: U^n momenta of the cell (own-cell Heun)
+    float d1;               // stage 2: the stage-1 dh of the cell (volume budget)
 };
 
 struct Src {                 // own-cell inputs of an output row
@@ -231,6 +232,7 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
         r.hraw = h;
         r.hun = hu;
         r.hvn = hv;
+        r.d1 = raw.dh;
         if (pick(tile_row_slot(L, sr), L.act0, L.act1, L.act2)) {
             // Integrate_1_k (grid.cu:856-922) of this cell from U^n and dU;
             // G is recomputed from U^n exactly as the stage-1 flux forms it
@@ -340,12 +342,23 @@ __device__ __forceinline__ void flush_tiles(const Dev& d, int s, int rr, const L
     }
 }
 
+// Per-lane sums of the volume budget over a run (m summed over cells; the
+// stage-1 outflow as a rate, m/s, weighted by the dt kernel).
+struct BudgetAcc {
+    float rain, drain, out, exch;
+};
+
 // Sources, the stage update and the stores of output row j (lanes of active
-// tiles); returns the cell's wet flag (stage 2).
-template <int Stage>
+// tiles); returns the cell's wet flag (stage 2).  With Budget, also the
+// cell's budget terms: outflow through the domain boundary faces of the cell
+// (fE, fw, gn, gs: mass fluxes of its east, west, north, south faces), and in
+// stage 2 the gridded rain, the drain of both stages and the exchange of the
+// dry clamps and the surge override of this step.
+template <int Stage, bool Budget>
 __device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, const Lane& L, int c,
                                            int j, const Row& C, const Src& src, float dh, float dhu,
-                                           float dhv, float bxw, float bxe, float bys, float byn)
+                                           float dhv, float bxw, float bxe, float bys, float byn,
+                                           float fE, float fw, float gn, float gs, BudgetAcc& acc)
 {
     const unsigned fl = S.fl;
     const size_t rbo = row_of(d, j);   // output row (uniform)
@@ -365,6 +378,33 @@ __device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, c
     }
     const bool srcc = (fl & RIFT_HYDROGRAPH) && c >= d.si0 && c <= d.si1 && j >= d.sj0 && j <= d.sj1;
     if (srcc) S0 += S.src_rate;
+    if constexpr (Budget) {
+        float outflow = 0.f;
+        if (c == 2) outflow -= fw * S.inv_dx;
+        if (c == d.nx - 3) outflow += fE * S.inv_dx;
+        if (j == 2) outflow -= gs * S.inv_dy;
+        if (j == d.ny - 3) outflow += gn * S.inv_dy;
+        if constexpr (Stage == 1) {
+            acc.out += outflow;
+        } else {
+            const float half_dt = 0.5f * S.dt;
+            acc.out = fmaf(half_dt, outflow, acc.out);
+            if (fl & RIFT_RAIN_GRID) acc.rain = fmaf(S.dt, src.rain, acc.rain);
+            // what the drain took in both stages: stage 1 capped with the
+            // stored depth and the previous dt, stage 2 with U*
+            float taken = 0.f;
+            const float cap1 = C.hraw * S.inv_dtp;
+            if (fl & RIFT_DRAIN_AVG) {
+                taken += (S.drain_rate > cap1) ? cap1 : S.drain_rate;
+                taken += (S.drain_rate > hC * S.inv_dt) ? hC * S.inv_dt : S.drain_rate;
+            }
+            if (fl & RIFT_DRAIN_GRID) {
+                taken += (src.drain > cap1) ? cap1 : src.drain;
+                taken += (src.drain > hC * S.inv_dt) ? hC * S.inv_dt : src.drain;
+            }
+            acc.drain = fmaf(half_dt, taken, acc.drain);
+        }
+    }
     float Fnew = 0.f;
     if (fl & RIFT_INFILTRATION) {
         if constexpr (Stage == 1) {
@@ -395,6 +435,11 @@ __device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, c
         const float hun = 0.5f * fmaf(dhu, S.dt, C.hun + C.hu);
         const float hvn = 0.5f * fmaf(dhv, S.dt, C.hvn + C.hv);
         const float hc = (hn > kEps) ? hn : 0.f;
+        if constexpr (Budget) {
+            // clamp of U^n, half of (stage-1 clamp + surge override) carried
+            // by U*, clamp of U^{n+1}
+            acc.exch += (h_n - C.hraw) + 0.5f * (C.h - fmaf(C.d1, S.dt, h_n)) + (hc - hn);
+        }
         // into the buffer the tile does not read
         const unsigned lo = L.c4 + (L.sel1 ? 0u : L.pl4);
         at(d.h + rbo, lo) = hc;
@@ -426,7 +471,15 @@ __device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, c
 // face behind C is carried from the previous row.  Crossing into another
 // tile row flushes the per-tile results and shifts the lane's tile-row
 // window.
-template <int Stage, bool Textbook, bool Up>
+// FP64 wave sum of one lane value, added to *dst by lane 0.
+__device__ __forceinline__ void wave_add_f64(double* dst, float v, int lane)
+{
+    double x = (double)v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (lane == 0 && x != 0.0) atomicAdd(dst, x);
+}
+
+template <int Stage, bool Textbook, bool Up, bool Budget>
 __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane)
 {
     constexpr int step = Up ? -1 : 1;
@@ -470,6 +523,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
 
     float mxl = 0.f, myl = 0.f;
     bool wetl = false;
+    BudgetAcc acc{0.f, 0.f, 0.f, 0.f};
     for (; j != jstop; j += step) {
         const int rj = (j - 2) / kTileH;   // uniform
         if (rj != rr) {
@@ -522,7 +576,8 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             myl = fmaxf(myl, gn.speed);
             // beds of the south / north faces of row j
             const float bys = byC, byn = Up ? byB : byD;
-            wetl |= output_row<Stage>(d, S, L, c, j, C, src, dh, dhu, dhv, bxw, bxe, bys, byn);
+            wetl |= output_row<Stage, Budget>(d, S, L, c, j, C, src, dh, dhu, dhv, bxw, bxe, bys, byn,
+                                              fE.f0, fw0, gn.f0, gs.f0, acc);
         }
         // roll the window
         if constexpr (Up) byB = byC;
@@ -536,6 +591,17 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         src = srcN;
     }
     flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane);
+    if constexpr (Budget) {
+        Scalars* sc = d.sc;
+        if constexpr (Stage == 1) {
+            wave_add_f64(&sc->b_flux1, acc.out, lane);
+        } else {
+            wave_add_f64(&sc->b_out, acc.out, lane);
+            wave_add_f64(&sc->b_exch, acc.exch, lane);
+            if (S.fl & RIFT_RAIN_GRID) wave_add_f64(&sc->b_rain_grid, acc.rain, lane);
+            if (S.fl & (RIFT_DRAIN_AVG | RIFT_DRAIN_GRID)) wave_add_f64(&sc->b_drain, acc.drain, lane);
+        }
+    }
 }
 
 // Work partition.  The launch holds exactly the waves that fit on the
@@ -555,7 +621,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
 // Otherwise: the active units (strip-major) form a sequence of rows and wave
 // i marches the i-th equal share of at least Dev::min_rows rows, downwards,
 // continuing from one unit into the next while they are vertically adjacent.
-template <int Stage, bool Textbook, int Waves>
+template <int Stage, bool Textbook, int Waves, bool Budget>
 __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
 {
     const int lane = threadIdx.x & 63;
@@ -599,8 +665,8 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         const int ja = 2 + (int)((long long)rows * mi / m);
         const int jb = 2 + (int)((long long)rows * (mi + 1) / m);
         if (ja >= jb) return;
-        if (mi & 1) march_run<Stage, Textbook, true>(d, S, s, ja, jb, lane);
-        else march_run<Stage, Textbook, false>(d, S, s, ja, jb, lane);
+        if (mi & 1) march_run<Stage, Textbook, true, Budget>(d, S, s, ja, jb, lane);
+        else march_run<Stage, Textbook, false, Budget>(d, S, s, ja, jb, lane);
         return;
     }
 
@@ -629,7 +695,7 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         const int rlast = last - s * d.nurows;
         const long long pe = min(pend, (long long)(kk + 1) * kTileH);
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (long long)kk * kTileH), d.ny - 2);
-        if (ja < jb) march_run<Stage, Textbook, false>(d, S, s, ja, jb, lane);
+        if (ja < jb) march_run<Stage, Textbook, false, Budget>(d, S, s, ja, jb, lane);
         k = kk + 1;
         a = 0;
         if ((long long)k * kTileH >= pend) break;
@@ -742,6 +808,10 @@ __global__ __launch_bounds__(256) void k_timestep2(Dev d)
         sc->wet_cells = cells_all;
         sc->wcu_total += cells_all;
         finish_timestep(d, vx_all, vy_all);
+        if (d.flags & RIFT_MASS_BUDGET) {
+            sc->b_out += 0.5 * sc->dt * sc->b_flux1;
+            sc->b_flux1 = 0.0;
+        }
         sc->mx_bits = 0u;
         sc->my_bits = 0u;
         sc->wet_cells_acc = 0ull;
@@ -778,10 +848,11 @@ void launch_grow2(const Dev& d, hipStream_t s)
 template <int Stage, int Waves>
 static void launch_march_w(const Dev& d, int blocks, hipStream_t s)
 {
-    if (d.flags & RIFT_TEXTBOOK)
-        hipLaunchKernelGGL((k_march<Stage, true, Waves>), dim3(blocks), dim3(256), 0, s, d);
-    else
-        hipLaunchKernelGGL((k_march<Stage, false, Waves>), dim3(blocks), dim3(256), 0, s, d);
+    const bool tb = d.flags & RIFT_TEXTBOOK, bud = d.flags & RIFT_MASS_BUDGET;
+    if (tb && bud) hipLaunchKernelGGL((k_march<Stage, true, Waves, true>), dim3(blocks), dim3(256), 0, s, d);
+    else if (tb) hipLaunchKernelGGL((k_march<Stage, true, Waves, false>), dim3(blocks), dim3(256), 0, s, d);
+    else if (bud) hipLaunchKernelGGL((k_march<Stage, false, Waves, true>), dim3(blocks), dim3(256), 0, s, d);
+    else hipLaunchKernelGGL((k_march<Stage, false, Waves, false>), dim3(blocks), dim3(256), 0, s, d);
 }
 
 void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks)
@@ -797,20 +868,23 @@ void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int block
 }
 
 template <int Stage, int Waves>
-static int resident(bool textbook, int cus)
+static int resident(bool textbook, bool budget, int cus)
 {
+    const void* k = textbook ? (budget ? (const void*)k_march<Stage, true, Waves, true>
+                                       : (const void*)k_march<Stage, true, Waves, false>)
+                             : (budget ? (const void*)k_march<Stage, false, Waves, true>
+                                       : (const void*)k_march<Stage, false, Waves, false>);
     int per_cu = 0;
-    hipError_t e = textbook
-        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_march<Stage, true, Waves>, 256, 0)
-        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_march<Stage, false, Waves>, 256, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     return per_cu * cus;
 }
 
-int march_resident_blocks(int stage, int variant, bool textbook, int cus)
+int march_resident_blocks(int stage, int variant, bool textbook, bool budget, int cus)
 {
-    if (stage == 1) return (variant & 1) ? resident<1, 4>(textbook, cus) : resident<1, 5>(textbook, cus);
-    return (variant & 2) ? resident<2, 3>(textbook, cus) : resident<2, 4>(textbook, cus);
+    if (stage == 1)
+        return (variant & 1) ? resident<1, 4>(textbook, budget, cus) : resident<1, 5>(textbook, budget, cus);
+    return (variant & 2) ? resident<2, 3>(textbook, budget, cus) : resident<2, 4>(textbook, budget, cus);
 }
 
 void launch_canonicalize(const Dev& d, hipStream_t s)

@@ -68,6 +68,7 @@ struct rift_grid {
     // counters clears it
     bool primed = false;
     int variant = 0;              // launch variant of the marching kernels
+    double budget_v0 = 0.0;       // stored + infiltrated volume at the budget reset, m^3
     int cus = 0;                  // compute units of the device
     int march_blocks[2][kMarchVariants] = {};   // resident workgroups per stage/variant
     bool per_phase = false;       // rift_step uses the one-kernel-per-phase path
@@ -292,7 +293,8 @@ int rift_create(const rift_desc* desc, rift_grid** out)
         for (int st = 0; st < 2; ++st)
             for (int v = 0; v < kMarchVariants; ++v)
                 g->march_blocks[st][v] =
-                    march_resident_blocks(st + 1, v, (desc->flags & RIFT_TEXTBOOK) != 0, g->cus);
+                    march_resident_blocks(st + 1, v, (desc->flags & RIFT_TEXTBOOK) != 0,
+                                          (desc->flags & RIFT_MASS_BUDGET) != 0, g->cus);
     }
     *out = g;
     return RIFT_OK;
@@ -414,7 +416,7 @@ int rift_init(rift_grid* g, double t0)
     HIPCHK(hipStreamSynchronize(g->stream));
     g->inited = true;
     g->primed = false;
-    return RIFT_OK;
+    return rift_budget_reset(g);
 }
 
 int rift_set_series(rift_grid* g, int which, const float* time_s, const float* value, int n)
@@ -997,8 +999,45 @@ int rift_mass_balance(rift_grid* g, double* out6)
     out6[1] = vf * area;
     out6[2] = sc.v_rain * area;
     out6[3] = sc.v_src * area;
-    out6[4] = sc.v_drain * area;
-    out6[5] = sc.v_out * area;
+    out6[4] = sc.b_drain * area;
+    out6[5] = sc.b_out * area;
+    return RIFT_OK;
+}
+
+int rift_budget_reset(rift_grid* g)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_budget_reset: null handle");
+    double m[6];
+    int rc = rift_mass_balance(g, m);
+    if (rc) return rc;
+    g->budget_v0 = m[0] + m[1];
+    const double zeros[5] = {0, 0, 0, 0, 0};
+    Scalars* sc = g->d.sc;
+    HIPCHK(hipMemcpyAsync(&sc->v_rain, zeros, 4 * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(&sc->b_rain_grid, zeros, 5 * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_budget(rift_grid* g, double* out9)
+{
+    if (!g || !out9) return fail(RIFT_E_ARG, "rift_budget: null argument");
+    double m[6];
+    int rc = rift_mass_balance(g, m);
+    if (rc) return rc;
+    Scalars sc;
+    HIPCHK(hipMemcpyAsync(&sc, g->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    const double area = g->d.dx * g->d.dy;
+    out9[0] = m[0];
+    out9[1] = m[1];
+    out9[2] = g->budget_v0;
+    out9[3] = m[2] + sc.b_rain_grid * area;
+    out9[4] = m[3];
+    out9[5] = sc.b_drain * area;
+    out9[6] = sc.b_out * area;
+    out9[7] = sc.b_exch * area;
+    out9[8] = (m[0] + m[1] - out9[2]) - (out9[3] + out9[4] - out9[5] - out9[6] + out9[7]);
     return RIFT_OK;
 }
 

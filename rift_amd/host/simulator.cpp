@@ -1,6 +1,7 @@
 // simulator.cpp -- RIFT's driver (simulator.cu) over the device step.
 #include "simulator.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -298,6 +299,7 @@ void Simulator::InitSimulation()
     if (tile_acceleration_) flags |= RIFT_TILE_ACCEL;
     if (h_init_) flags |= RIFT_H_INIT;
     if (rain_interp_) flags |= RIFT_RAIN_INTERP;
+    if (check_volume_) flags |= RIFT_MASS_BUDGET;
     rift_desc d{};
     d.nx = gc_.h_nx;
     d.ny = gc_.h_ny;
@@ -405,10 +407,13 @@ void Simulator::PrintData()
         write_grid(output_dir + "/q" + std::to_string(count_print) + ".txt", q.data(), gc_);
     }
     if (check_volume_) {
-        double mb[6];
-        check(rift_mass_balance(grid_, mb), "mass balance");
-        std::cout << "volume stored " << mb[0] << " infiltrated " << mb[1] << " added "
-                  << mb[2] + mb[3] + V_added_ * gc_.h_dx * gc_.h_dy << std::endl;
+        // the FP64 device budget (m^3) and its relative closure
+        double b[9];
+        check(rift_budget(grid_, b), "volume budget");
+        const double scale = std::max({std::fabs(b[0]) + std::fabs(b[1]), std::fabs(b[2]), 1e-300});
+        std::cout << "volume stored " << b[0] << " infiltrated " << b[1] << " rain " << b[3]
+                  << " source " << b[4] << " drain " << b[5] << " outflow " << b[6] << " exchange "
+                  << b[7] << " residual " << b[8] << " (" << b[8] / scale << ")" << std::endl;
     }
     ++count_print;
 }

@@ -35,6 +35,7 @@ FIELDS = ["w", "hu", "hv", "depth", "bc", "n", "K", "F", "dw", "dhu", "dhv", "G"
           "h_max", "q_max", "t_wet", "t_peak", "t_dry", "surge", "rainmap", "drainmap", "q"]
 SER_HYETO, SER_DRAIN, SER_HYDRO = 0, 1, 2
 MAP_RAIN, MAP_DRAIN, MAP_SURGE = 0, 1, 2
+MASS_BUDGET = 1 << 13
 OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT = 1, 2, 3
 NVARIANTS = 16
 KERNEL_CLASSES = ["flux1", "stage2_flux", "integrate", "boundaries", "grow", "timestep"]
@@ -89,6 +90,8 @@ SIGNATURES = [
     ("rift_get_rates", C.c_int, [_VP, _F]),
     ("rift_sum_interior", C.c_int, [_VP, C.c_int, _D]),
     ("rift_mass_balance", C.c_int, [_VP, _D]),
+    ("rift_budget", C.c_int, [_VP, _D]),
+    ("rift_budget_reset", C.c_int, [_VP]),
     ("rift_get_stream", C.c_int, [_VP, C.POINTER(_VP)]),
     ("rift_profile_enable", C.c_int, [_VP, C.c_int]),
     ("rift_kernel_stats", C.c_int, [_VP, C.c_int, C.POINTER(C.c_long), _D]),
@@ -334,6 +337,16 @@ class Grid:
         out = C.c_double()
         _check(load().rift_sum_interior(self.h, FIELDS.index(name), C.byref(out)), "rift_sum_interior")
         return out.value
+
+    def budget(self):
+        """[stored, infiltrated, total at reset, rain, source, drain, outflow,
+        exchange, residual] in m^3 (rift_budget)."""
+        out = np.empty(9)
+        _check(load().rift_budget(self.h, _dp(out)), "rift_budget")
+        return out
+
+    def budget_reset(self):
+        _check(load().rift_budget_reset(self.h), "rift_budget_reset")
 
     def mass_balance(self):
         out = np.empty(6)

@@ -99,6 +99,8 @@ def lib(fp32=False):
     L.orc_get_tile_speeds.argtypes = [C.c_void_p, _D, _D]
     L.orc_get_rates.argtypes = [C.c_void_p, _F]
     L.orc_mass.argtypes = [C.c_void_p, _D]
+    L.orc_budget.argtypes = [C.c_void_p, _D]
+    L.orc_budget_reset.argtypes = [C.c_void_p]
     _LIBS[key] = L
     return L
 
@@ -217,6 +219,16 @@ class Oracle:
         out = np.empty(4)
         self.L.orc_mass(self.s, dptr(out))
         return out
+
+    def budget(self):
+        """[stored, infiltrated, total at reset, rain, source, drain, outflow,
+        exchange, residual] in m^3."""
+        out = np.empty(9)
+        self.L.orc_budget(self.s, dptr(out))
+        return out
+
+    def budget_reset(self):
+        self.L.orc_budget_reset(self.s)
 
     def depth(self):
         return self.field("depth")

@@ -279,3 +279,45 @@ def test_launch_variants_bit_identical_and_autotuner(rift):
     np.testing.assert_array_equal(g.field("depth"), before[1])
     g.step(20)   # continues with the chosen variant
     g.close()
+
+
+@pytest.mark.parametrize("accel", [True, False])
+def test_device_volume_budget(rift, accel):
+    """The device's FP64 volume budget (rift_budget, RIFT_MASS_BUDGET) closes
+    to FP32 rounding of the state and matches the oracle's FP64 budget term
+    by term (rain, source, drain, outflow through the outflow boundaries)."""
+    n = 48
+    N = n + 1
+    dem = 6.0 * synth.fractal_dem(n, 0.6, 21) + np.linspace(0, 3, N)[None, :]
+    h0 = np.maximum(3.5 - dem, 0.0)
+    K = 5.0 * np.exp(0.5 * synth.normal(5, N * N).reshape(N, N))
+    base_o = O.RAIN_AVG | O.DRAIN_AVG | O.HYDROGRAPH | (O.TILE_ACCEL if accel else 0)
+    o = O.Oracle(dem, 2.0, h0=h0, n_const=0.03, K=K, psi=0.11, dtheta=0.3, flags=base_o)
+    gflags = (rift.RAIN_AVG | rift.DRAIN_AVG | rift.HYDROGRAPH | rift.MASS_BUDGET |
+              (rift.TILE_ACCEL if accel else 0))
+    g = rift.grid_from_rasters(dem, 2.0, h0=h0, n_const=0.03, K=K, psi=0.11, dtheta=0.3,
+                               flags=gflags)
+    series = {O.SER_HYETO: ([0, 0.01, 0.05], [50, 120, 0]), O.SER_DRAIN: ([0, 0.05], [10, 10]),
+              O.SER_HYDRO: ([0, 0.01], [20, 30])}
+    idx = [j * o.nx + i for i in range(20, 23) for j in range(30, 32)]
+    o.set_sources(idx)
+    g.set_source_cells(idx)
+    for which, (hours, vals) in series.items():
+        hrs, vv = np.array(hours, np.float32), np.array(vals, np.float32)
+        o.set_series_hours(which, hrs, vv)
+        t, v = synth.series_hours_to_seconds(hrs, vv)
+        g.set_series(which, t, v)
+    for _ in range(4):
+        o.step(40)
+        g.step(40)
+        bg, bo = g.budget(), o.budget()
+        scale = np.abs(bg[:8]).max()
+        assert abs(bg[8]) <= 1e-6 * scale, bg
+    names = ["stored", "infiltrated", "v0", "rain", "source", "drain", "outflow"]
+    tol = {"stored": 1e-4, "infiltrated": 1e-4, "v0": 1e-9, "rain": 1e-4, "source": 1e-4,
+           "drain": 2e-3, "outflow": 2e-3}
+    for k, name in enumerate(names):
+        assert abs(bg[k] - bo[k]) <= tol[name] * max(abs(bo[k]), 1e-9), (name, bg[k], bo[k])
+    # the exchange (dry clamps) is small here in both
+    assert abs(bg[7] - bo[7]) <= 1e-3 * scale
+    g.close()

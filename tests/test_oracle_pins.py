@@ -22,6 +22,7 @@ import pytest
 from scipy.optimize import brentq
 
 import oracle_lib as O
+from rift_amd import synth
 
 G = float(np.float32(9.80665))
 EPS = float(np.float32(1.19209e-07))
@@ -388,3 +389,45 @@ def test_fp32_baseline_tracks_fp64():
     b.step(40)
     assert abs(a.time[0] - b.time[0]) < 1e-5 * a.time[0]
     assert np.abs(a.interior(a.depth()) - b.interior(b.depth())).max() < 1e-4
+
+
+# --- FP64 volume budget ------------------------------------------------------
+
+def _budget_case(flags, surge=False, n=48):
+    N = n + 1
+    dem = 6.0 * synth.fractal_dem(n, 0.6, 21) + np.linspace(0, 3, N)[None, :]
+    h0 = np.maximum(3.5 - dem, 0.0)
+    K = 5.0 * np.exp(0.5 * synth.normal(5, N * N).reshape(N, N))
+    o = O.Oracle(dem, 2.0, h0=h0, n_const=0.03, K=K, psi=0.11, dtheta=0.3, flags=flags)
+    o.set_series_hours(O.SER_HYETO, np.array([0, 0.01, 0.05], np.float32),
+                       np.array([50, 120, 0], np.float32))
+    o.set_series_hours(O.SER_DRAIN, np.array([0, 0.05], np.float32), np.array([10, 10], np.float32))
+    o.set_sources([j * o.nx + i for i in range(20, 23) for j in range(30, 32)])
+    o.set_series_hours(O.SER_HYDRO, np.array([0, 0.01], np.float32), np.array([20, 30], np.float32))
+    if surge:
+        maps = np.full((4, N, N), -9999.0)
+        for k in range(4):
+            maps[k, :, :3] = 0.5 + 0.4 * k     # surge depth on the west edge
+        o.set_map_series(O.MAP_SURGE, maps, 10, 40.0)
+    return o
+
+
+@pytest.mark.parametrize("case", ["heun", "surge", "euler", "no_accel"])
+def test_oracle_volume_budget_closes(case):
+    """The change of stored + infiltrated volume equals rain + source - drain
+    - outflow + exchange to FP64 rounding; each term is exercised."""
+    base = O.RAIN_AVG | O.DRAIN_AVG | O.HYDROGRAPH
+    flags = {"heun": O.TILE_ACCEL | base, "surge": O.TILE_ACCEL | base | O.SURGE_GRID,
+             "euler": O.TILE_ACCEL | base | O.EULER, "no_accel": base}[case]
+    o = _budget_case(flags, surge=(case == "surge"))
+    for _ in range(4):
+        o.step(40)
+        b = o.budget()
+        scale = np.abs(b[:8]).max()
+        assert abs(b[8]) <= 1e-12 * scale, (case, b)
+    stored, infil, v0, rain, src, drain, out, exch, _ = b
+    assert rain > 0 and src > 0 and drain > 0 and out > 0 and infil > 0
+    if case == "surge":
+        assert abs(exch) > 1e-3 * scale
+    # a dropped term would not close: the outflow alone is far above rounding
+    assert out > 1e6 * abs(b[8])
