@@ -246,6 +246,34 @@ int rift_kernel_stats(rift_grid* g, int kernel, long* launches, double* total_ms
  * cumulated since rift_init, read back from the device. */
 int rift_wet_cell_updates(rift_grid* g, long* last_step, long* total);
 
+/* ---- Ensembles (no reference counterpart: the reference holds one domain
+ * in process-global state, grid.cu:21-40) ----------------------------------
+ * Many domains (members) of different sizes, inputs and forcing advanced
+ * together: each phase of a step is one launch over all members, and the
+ * marching kernels deal equal row shares of every member's active units to
+ * the device's resident waves.  Each member keeps its own t, dt (CFL of its
+ * own speeds), forcing and tile sets, and its cells see exactly the
+ * arithmetic of a single-domain step (bit-identical results).  Members are
+ * ordinary handles for set-up and field access (rift_ensemble_member); they
+ * share the ensemble's stream.  They must agree on RIFT_TEXTBOOK,
+ * RIFT_MASS_BUDGET and RIFT_EULER (Euler is not supported), and use one
+ * device. */
+typedef struct rift_ensemble rift_ensemble;
+int rift_ensemble_create(int n, const rift_desc* descs, rift_ensemble** out);
+int rift_ensemble_destroy(rift_ensemble* e);
+int rift_ensemble_size(rift_ensemble* e);
+rift_grid* rift_ensemble_member(rift_ensemble* e, int i);
+/* End time of member i: from the step whose start time is >= tf on, the
+ * member is left as it is (0 = never). */
+int rift_ensemble_set_tf(rift_ensemble* e, int i, double tf);
+/* nsteps batched steps: one step of every member that has not ended. */
+int rift_ensemble_step(rift_ensemble* e, int nsteps);
+/* Members whose time has reached their end time (blocking). */
+int rift_ensemble_finished(rift_ensemble* e, int* n_finished);
+/* Steps until every member has ended (each needs an end time), checking
+ * every check_every steps; max_steps < 0: no limit. */
+int rift_ensemble_run(rift_ensemble* e, int check_every, long max_steps, long* steps_done);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,7 @@ struct Scalars {
     // exchange; b_flux1 = stage-1 boundary outflow rate (m/s summed over
     // cells), weighted with dt/2 by the dt kernel
     double b_rain_grid, b_drain, b_out, b_exch, b_flux1;
+    int finished;                     // the domain reached its end time (ensembles)
     float nxt_rain, nxt_drain, nxt_hydro, nxt_cell;
 };
 
@@ -74,6 +75,7 @@ struct Dev {
     long long soff;              // h0 - h (elements): offset of the second buffer
     int n_uniform;
     float n_const;
+    double tf;                   // end time of an ensemble member (0: none)
     int band;                    // XCD-banded order of the marching shares
     int min_rows;                // least rows per marching share
     float *mx, *my;              // per-tile maxima of the last flux pass
@@ -185,6 +187,12 @@ void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int block
 // workgroups of a marching kernel resident at once on a device of `cus` CUs
 int march_resident_blocks(int stage, int variant, bool textbook, bool budget, int cus);
 constexpr int kMarchVariants = 4;
+// Ensembles (devs: device array of the members' Dev)
+void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s);
+void launch_timestep2_ens(const Dev* devs, int nmem, int max_tiles, hipStream_t s);
+void launch_march_ens(const Dev* devs, int nmem, unsigned flags, int stage, int min_rows, int blocks,
+                      hipStream_t s);
+int march_ens_resident_blocks(int stage, bool textbook, bool budget, int cus);
 void launch_canonicalize(const Dev& d, hipStream_t s);
 
 }  // namespace rift

@@ -75,11 +75,19 @@ __device__ __forceinline__ void strip_tiles(const Dev& d, int s, int& txa, int& 
 // kernels walk units, and the dt kernel counts the active tiles and cells.
 // Thread 0 installs the forcing the last dt kernel evaluated for this step.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grow2(Dev d)
+// A domain with an end time (ensemble members) stops stepping once its time
+// has reached it: no units, no dt.
+__device__ __forceinline__ bool finished(const Dev& d)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    return d.tf > 0.0 && d.sc->t >= d.tf;
+}
+
+__device__ __forceinline__ void grow2_body(const Dev& d, int t)
+{
+    const bool done = finished(d);
     if (t == 0) {
         Scalars* sc = d.sc;
+        sc->finished = done ? 1 : 0;
         if (sc->next_valid) {
             sc->rain_rate = sc->nxt_rain;
             sc->drain_rate = sc->nxt_drain;
@@ -105,7 +113,7 @@ __global__ __launch_bounds__(256) void k_grow2(Dev d)
             d.my1[t] = 0.f;
         }
     }
-    if (is_unit) {
+    if (is_unit && !done) {
         // units are numbered strip-major: u = strip * nurows + tile row
         const int u = t - d.ntiles;
         const int s = u / d.nurows, r = u - s * d.nurows;
@@ -122,6 +130,11 @@ __global__ __launch_bounds__(256) void k_grow2(Dev d)
     if (lane == 0 && mu) ubase = atomicAdd(&d.sc->n_units_acc, __popcll(mu));
     ubase = __shfl(ubase, 0);
     if (ua) d.ulist[ubase + __popcll(mu & lt)] = t - d.ntiles;
+}
+
+__global__ __launch_bounds__(256) void k_grow2(Dev d)
+{
+    grow2_body(d, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -747,7 +760,7 @@ __global__ __launch_bounds__(256) void k_clear_sel(Dev d)
 // simulator.cu:780-784, with the t of this step).
 constexpr int kDtBlocks = 32;
 
-__global__ __launch_bounds__(256) void k_timestep2(Dev d)
+__device__ __forceinline__ void timestep2_body(const Dev& d)
 {
     __shared__ float red[2][4];
     __shared__ unsigned long long redc[4];
@@ -799,7 +812,18 @@ __global__ __launch_bounds__(256) void k_timestep2(Dev d)
         last = atomicAdd(&sc->ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
+    if (last && threadIdx.x == 0 && sc->finished) {
+        // an ended domain keeps its time; the accumulators are cleared
+        sc->mx_bits = 0u;
+        sc->my_bits = 0u;
+        sc->wet_cells_acc = 0ull;
+        sc->n_active_acc = 0;
+        sc->ticket = 0u;
+        sc->n_units = 0;
+        sc->n_units_acc = 0;
+        sc->next_valid = 0;
+        sc->b_flux1 = 0.0;
+    } else if (last && threadIdx.x == 0) {
         __threadfence();
         const float vx_all = __uint_as_float(atomicAdd(&sc->mx_bits, 0u));
         const float vy_all = __uint_as_float(atomicAdd(&sc->my_bits, 0u));
@@ -826,6 +850,11 @@ __global__ __launch_bounds__(256) void k_timestep2(Dev d)
         sc->nxt_cell = f.cell;
         sc->next_valid = 1;
     }
+}
+
+__global__ __launch_bounds__(256) void k_timestep2(Dev d)
+{
+    timestep2_body(d);
 }
 
 void launch_timestep2(const Dev& d, hipStream_t s)
@@ -895,6 +924,159 @@ void launch_canonicalize(const Dev& d, hipStream_t s)
     const long total = (long)rows * d.P;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, d, rows);
     hipLaunchKernelGGL(k_clear_sel, dim3(tb), dim3(256), 0, s, d);
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: many domains (scenarios) advanced together, one launch per
+// phase.  devs[] holds every member's Dev in device memory; grow and dt run
+// one grid row per member (blockIdx.y); the marching kernels deal equal row
+// shares of the members' active units, in member order, to the resident
+// waves, so small and large members share the device without a launch each.
+// Every member keeps its own time, dt, forcing and tile sets; a member's
+// cells see exactly the arithmetic of a single-domain step.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grow2_ens(const Dev* __restrict__ devs)
+{
+    grow2_body(devs[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_timestep2_ens(const Dev* __restrict__ devs)
+{
+    timestep2_body(devs[blockIdx.y]);
+}
+
+__device__ __forceinline__ StepScalars step_scalars(const Dev& d)
+{
+    StepScalars S;
+    S.dt = (float)d.sc->dt;
+    S.inv_dt = 1.f / S.dt;
+    S.dtp = (float)d.sc->dt_prev;
+    S.inv_dtp = 1.f / S.dtp;
+    S.kappa = d.kappa;
+    S.fl = d.flags;
+    S.rain_rate = d.sc->rain_rate;
+    S.drain_rate = d.sc->drain_rate;
+    S.src_rate = d.sc->src_cell_rate;
+    S.tnow = (float)d.sc->t;
+    S.inv_dx = d.inv_dx;
+    S.inv_dy = d.inv_dy;
+    return S;
+}
+
+template <int Stage>
+__device__ __forceinline__ int member_units(const Dev& d)
+{
+    return min((Stage == 1) ? d.sc->n_units_acc : d.sc->n_units, d.nunits);
+}
+
+template <int Stage, bool Textbook, int Waves, bool Budget>
+__global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict__ devs, int nmem, int min_rows)
+{
+    const int lane = threadIdx.x & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    long long total = 0;
+    for (int m = 0; m < nmem; ++m) total += (long long)member_units<Stage>(devs[m]) * kTileH;
+    if (total == 0) return;
+    const int G = (int)gridDim.x;
+    const int nb = min(G, 8), x = (int)blockIdx.x % nb, q = G / nb, r = G % nb;
+    const int pos = x * q + min(x, r) + (int)blockIdx.x / nb;
+    const long long W = 4LL * G;
+    const long long wp = 4LL * pos + wid;
+    const long long nshare = max(1LL, min(W, total / max(min_rows, 1)));
+    const long long i0 = (wp * nshare + W - 1) / W, i1 = ((wp + 1) * nshare + W - 1) / W;
+    if (i0 >= i1) return;
+    const long long p0 = total * i0 / nshare, pend = total * i1 / nshare;
+    if (p0 >= pend) return;
+
+    // member of the first unit of the share
+    long long k = p0 / kTileH;
+    int a = (int)(p0 - k * kTileH);
+    int mem = 0;
+    long long base = 0;
+    int cnt = member_units<Stage>(devs[0]);
+    while (k >= base + cnt) {
+        base += cnt;
+        ++mem;
+        cnt = member_units<Stage>(devs[mem]);
+    }
+    for (;;) {
+        const Dev& d = devs[mem];
+        const bool dense = cnt == d.nunits;
+        const StepScalars S = step_scalars(d);
+        // runs of vertically adjacent units of one strip within this member
+        const int kl = (int)(k - base);
+        const int u = __builtin_amdgcn_readfirstlane(dense ? kl : d.ulist[kl]);
+        const int s = u / d.nurows, rr = u - s * d.nurows;
+        const int ja = rr * kTileH + 2 + a;
+        int last = u, kk = kl;
+        while (kk + 1 < cnt && (base + kk + 1) * kTileH < pend) {
+            const int un = __builtin_amdgcn_readfirstlane(dense ? kk + 1 : d.ulist[kk + 1]);
+            if (un != last + 1 || un - s * d.nurows >= d.nurows) break;
+            last = un;
+            ++kk;
+        }
+        const int rlast = last - s * d.nurows;
+        const long long pe = min(pend, (base + kk + 1) * kTileH);
+        const int jb = min(rlast * kTileH + 2 + (int)(pe - (base + kk) * kTileH), d.ny - 2);
+        if (ja < jb) march_run<Stage, Textbook, false, Budget>(d, S, s, ja, jb, lane);
+        k = base + kk + 1;
+        a = 0;
+        if (k * kTileH >= pend) break;
+        while (k >= base + cnt) {   // next member with units
+            base += cnt;
+            ++mem;
+            cnt = member_units<Stage>(devs[mem]);
+        }
+    }
+}
+
+void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_grow2_ens, dim3((max_threads + 255) / 256, nmem), dim3(256), 0, s, devs);
+}
+
+void launch_timestep2_ens(const Dev* devs, int nmem, int max_tiles, hipStream_t s)
+{
+    const int blocks = std::max(1, std::min(kDtBlocks, (max_tiles + 255) / 256));
+    hipLaunchKernelGGL(k_timestep2_ens, dim3(blocks, nmem), dim3(256), 0, s, devs);
+}
+
+template <int Stage, int Waves>
+static void launch_march_ens_w(const Dev* devs, int nmem, unsigned flags, int min_rows, int blocks,
+                               hipStream_t s)
+{
+    const bool tb = flags & RIFT_TEXTBOOK, bud = flags & RIFT_MASS_BUDGET;
+    const dim3 g(blocks), b(256);
+    if (tb && bud) hipLaunchKernelGGL((k_march_ens<Stage, true, Waves, true>), g, b, 0, s, devs, nmem, min_rows);
+    else if (tb) hipLaunchKernelGGL((k_march_ens<Stage, true, Waves, false>), g, b, 0, s, devs, nmem, min_rows);
+    else if (bud) hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, true>), g, b, 0, s, devs, nmem, min_rows);
+    else hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, false>), g, b, 0, s, devs, nmem, min_rows);
+}
+
+void launch_march_ens(const Dev* devs, int nmem, unsigned flags, int stage, int min_rows, int blocks,
+                      hipStream_t s)
+{
+    blocks = std::max(1, blocks);
+    if (stage == 1) launch_march_ens_w<1, 5>(devs, nmem, flags, min_rows, blocks, s);
+    else launch_march_ens_w<2, 4>(devs, nmem, flags, min_rows, blocks, s);
+}
+
+template <int Stage, int Waves>
+static int resident_ens(bool textbook, bool budget, int cus)
+{
+    const void* k = textbook ? (budget ? (const void*)k_march_ens<Stage, true, Waves, true>
+                                       : (const void*)k_march_ens<Stage, true, Waves, false>)
+                             : (budget ? (const void*)k_march_ens<Stage, false, Waves, true>
+                                       : (const void*)k_march_ens<Stage, false, Waves, false>);
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    return per_cu * cus;
+}
+
+int march_ens_resident_blocks(int stage, bool textbook, bool budget, int cus)
+{
+    return stage == 1 ? resident_ens<1, 5>(textbook, budget, cus) : resident_ens<2, 4>(textbook, budget, cus);
 }
 
 }  // namespace rift

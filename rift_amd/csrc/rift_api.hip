@@ -22,6 +22,8 @@ using namespace rift;
 namespace {
 
 thread_local std::string g_last_error;
+// set while an ensemble creates its members: they share its stream
+thread_local hipStream_t g_member_stream = nullptr;
 
 int fail(int code, const std::string& msg)
 {
@@ -68,6 +70,7 @@ struct rift_grid {
     // counters clears it
     bool primed = false;
     int variant = 0;              // launch variant of the marching kernels
+    bool own_stream = true;       // false for ensemble members (the ensemble's stream)
     double budget_v0 = 0.0;       // stored + infiltrated volume at the budget reset, m^3
     int cus = 0;                  // compute units of the device
     int march_blocks[2][kMarchVariants] = {};   // resident workgroups per stage/variant
@@ -233,10 +236,15 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.band = 1;
     d.min_rows = kTileH;
 
-    hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete g;
-        return fail(RIFT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    if (g_member_stream) {
+        g->stream = g_member_stream;
+        g->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete g;
+            return fail(RIFT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
     }
     const unsigned f = desc->flags;
     bool ok = true;
@@ -311,7 +319,7 @@ int rift_destroy(rift_grid* g)
     }
     for (hipEvent_t e : g->event_pool) (void)hipEventDestroy(e);
     for (void* p : g->allocs) (void)hipFree(p);
-    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->stream && g->own_stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return RIFT_OK;
 }
@@ -1088,6 +1096,157 @@ int rift_wet_cell_updates(rift_grid* g, long* last_step, long* total)
     HIPCHK(hipStreamSynchronize(g->stream));
     *last_step = (long)c[0];
     *total = (long)c[1];
+    return RIFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct rift_ensemble {
+    std::vector<rift_grid*> members;
+    hipStream_t stream = nullptr;
+    Dev* devs = nullptr;            // device copy of the members' Dev
+    std::vector<Dev> host_devs;
+    unsigned flags = 0;             // TEXTBOOK / MASS_BUDGET (common to all members)
+    int march_blocks[2] = {0, 0};
+    int max_threads = 0, max_tiles = 0;
+    int min_rows = kTileH;
+};
+
+extern "C" {
+
+int rift_ensemble_create(int n, const rift_desc* descs, rift_ensemble** out)
+{
+    if (n < 1 || !descs || !out) return fail(RIFT_E_ARG, "rift_ensemble_create: bad argument");
+    *out = nullptr;
+    const unsigned common = RIFT_TEXTBOOK | RIFT_MASS_BUDGET | RIFT_EULER;
+    for (int i = 0; i < n; ++i) {
+        if (descs[i].device != descs[0].device)
+            return fail(RIFT_E_UNSUPPORTED, "rift_ensemble_create: members must share one device");
+        if ((descs[i].flags & common) != (descs[0].flags & common))
+            return fail(RIFT_E_UNSUPPORTED,
+                        "rift_ensemble_create: members must agree on TEXTBOOK, MASS_BUDGET and EULER");
+    }
+    if (descs[0].flags & RIFT_EULER)
+        return fail(RIFT_E_UNSUPPORTED, "rift_ensemble_create: Euler members run one at a time");
+    HIPCHK(hipSetDevice(descs[0].device));
+    rift_ensemble* e = new rift_ensemble();
+    hipError_t he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete e;
+        return fail(RIFT_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    e->flags = descs[0].flags & common;
+    int rc = RIFT_OK;
+    g_member_stream = e->stream;
+    for (int i = 0; i < n && rc == RIFT_OK; ++i) {
+        rift_grid* g = nullptr;
+        rc = rift_create(&descs[i], &g);
+        if (rc == RIFT_OK) e->members.push_back(g);
+    }
+    g_member_stream = nullptr;
+    if (rc == RIFT_OK && hipMalloc(&e->devs, sizeof(Dev) * n) != hipSuccess)
+        rc = fail(RIFT_E_HIP, "rift_ensemble_create: device allocation failed");
+    if (rc != RIFT_OK) {
+        rift_ensemble_destroy(e);
+        return rc;
+    }
+    for (rift_grid* g : e->members) {
+        e->max_threads = std::max(e->max_threads, g->d.ntiles + g->d.nunits);
+        e->max_tiles = std::max(e->max_tiles, g->d.ntiles);
+    }
+    const int cus = e->members[0]->cus;
+    for (int st = 0; st < 2; ++st)
+        e->march_blocks[st] = march_ens_resident_blocks(st + 1, (e->flags & RIFT_TEXTBOOK) != 0,
+                                                        (e->flags & RIFT_MASS_BUDGET) != 0, cus);
+    *out = e;
+    return RIFT_OK;
+}
+
+rift_grid* rift_ensemble_member(rift_ensemble* e, int i)
+{
+    if (!e || i < 0 || i >= (int)e->members.size()) return nullptr;
+    return e->members[i];
+}
+
+int rift_ensemble_size(rift_ensemble* e) { return e ? (int)e->members.size() : 0; }
+
+int rift_ensemble_set_tf(rift_ensemble* e, int i, double tf)
+{
+    if (!e || i < 0 || i >= (int)e->members.size())
+        return fail(RIFT_E_ARG, "rift_ensemble_set_tf: bad argument");
+    e->members[i]->d.tf = tf;
+    return RIFT_OK;
+}
+
+int rift_ensemble_step(rift_ensemble* e, int nsteps)
+{
+    if (!e || nsteps < 0) return fail(RIFT_E_ARG, "rift_ensemble_step: bad argument");
+    const int n = (int)e->members.size();
+    e->host_devs.resize(n);
+    for (int i = 0; i < n; ++i) {
+        rift_grid* g = e->members[i];
+        if (!g->inited) return fail(RIFT_E_STATE, "rift_ensemble_step: member not initialised");
+        if (g->per_phase) return fail(RIFT_E_UNSUPPORTED, "rift_ensemble_step: member on the per-phase path");
+        prime(g);
+        e->host_devs[i] = g->d;
+    }
+    HIPCHK(hipMemcpyAsync(e->devs, e->host_devs.data(), sizeof(Dev) * n, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));   // host_devs may change before the next call
+    for (int k = 0; k < nsteps; ++k) {
+        for (rift_grid* g : e->members)
+            if (g->d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(g->d, e->stream);
+        launch_grow2_ens(e->devs, n, e->max_threads, e->stream);
+        launch_march_ens(e->devs, n, e->flags, 1, e->min_rows, e->march_blocks[0], e->stream);
+        launch_timestep2_ens(e->devs, n, e->max_tiles, e->stream);
+        launch_march_ens(e->devs, n, e->flags, 2, e->min_rows, e->march_blocks[1], e->stream);
+    }
+    for (rift_grid* g : e->members) g->scattered = true;
+    return check_launch("rift_ensemble_step");
+}
+
+int rift_ensemble_finished(rift_ensemble* e, int* n_finished)
+{
+    if (!e || !n_finished) return fail(RIFT_E_ARG, "rift_ensemble_finished: null argument");
+    int done = 0;
+    for (rift_grid* g : e->members) {
+        double t = 0.0;
+        HIPCHK(hipMemcpyAsync(&t, &g->d.sc->t, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        if (g->d.tf > 0.0 && t >= g->d.tf) ++done;
+    }
+    *n_finished = done;
+    return RIFT_OK;
+}
+
+int rift_ensemble_run(rift_ensemble* e, int check_every, long max_steps, long* steps_done)
+{
+    if (!e || check_every < 1) return fail(RIFT_E_ARG, "rift_ensemble_run: bad argument");
+    for (rift_grid* g : e->members)
+        if (!(g->d.tf > 0.0)) return fail(RIFT_E_STATE, "rift_ensemble_run: every member needs an end time");
+    long done_steps = 0;
+    int nfin = 0;
+    int rc = rift_ensemble_finished(e, &nfin);
+    while (rc == RIFT_OK && nfin < (int)e->members.size() && (max_steps < 0 || done_steps < max_steps)) {
+        const long k = (max_steps < 0) ? check_every : std::min<long>(check_every, max_steps - done_steps);
+        if ((rc = rift_ensemble_step(e, (int)k))) break;
+        done_steps += k;
+        rc = rift_ensemble_finished(e, &nfin);
+    }
+    if (steps_done) *steps_done = done_steps;
+    return rc;
+}
+
+int rift_ensemble_destroy(rift_ensemble* e)
+{
+    if (!e) return RIFT_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (rift_grid* g : e->members) rift_destroy(g);
+    if (e->devs) (void)hipFree(e->devs);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
     return RIFT_OK;
 }
 

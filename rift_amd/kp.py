@@ -101,6 +101,14 @@ SIGNATURES = [
     ("rift_read_series", C.c_int, [C.c_char_p, _F, _F, C.c_int, _I]),
     ("rift_read_raster", C.c_int, [C.c_char_p, C.c_int, _I, _I, _D, _D, C.c_long]),
     ("rift_write_grid", C.c_int, [C.c_char_p, _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]),
+    ("rift_ensemble_create", C.c_int, [C.c_int, C.POINTER(RiftDesc), C.POINTER(_VP)]),
+    ("rift_ensemble_destroy", C.c_int, [_VP]),
+    ("rift_ensemble_size", C.c_int, [_VP]),
+    ("rift_ensemble_member", _VP, [_VP, C.c_int]),
+    ("rift_ensemble_set_tf", C.c_int, [_VP, C.c_int, C.c_double]),
+    ("rift_ensemble_step", C.c_int, [_VP, C.c_int]),
+    ("rift_ensemble_finished", C.c_int, [_VP, _I]),
+    ("rift_ensemble_run", C.c_int, [_VP, C.c_int, C.c_long, C.POINTER(C.c_long)]),
     ("rift_config_open", C.c_int, [C.c_char_p, C.POINTER(_VP)]),
     ("rift_config_close", None, [_VP]),
     ("rift_config_has", C.c_int, [_VP, C.c_char_p]),
@@ -434,6 +442,72 @@ def write_raster(path, data, xll=0.0, yll=0.0, cellsize=1.0, nodata=-9999.0, fmt
                 f"cellsize {cellsize!r}\nNODATA_value {nodata!r}\n")
         for j in range(nr - 1, -1, -1):
             f.write(" ".join(fmt % v for v in data[j]) + "\n")
+
+
+class Ensemble:
+    """Many domains stepped together (rift_ensemble_*).  `members` are
+    dicts of grid_from_rasters arguments (dem, dx, n_const/n, K, psi, dtheta,
+    flags, h0, t0); each member is reachable as a borrowed Grid for set-up
+    and field access."""
+
+    def __init__(self, members):
+        L = load()
+        descs, geos, self._kw = [], [], []
+        for m in members:
+            m = dict(m)
+            flags = m.get("flags", TILE_ACCEL)
+            if m.get("h0") is not None:
+                flags |= H_INIT
+            if m.get("K") is not None:
+                flags |= INFILTRATION
+            geo = geometry(m["dem"], h0=m.get("h0"), n=m.get("n"), n_const=m.get("n_const", 0.0),
+                           K=m.get("K"))
+            dx = m["dx"]
+            descs.append(RiftDesc(geo["nx"], geo["ny"], dx, dx, L.rift_kappa(dx, dx),
+                                  m.get("courant", 0.25), m.get("psi", 0.0), m.get("dtheta", 0.0),
+                                  -9999.0, flags, m.get("device", 0)))
+            geos.append(geo)
+            self._kw.append(m)
+        arr = (RiftDesc * len(descs))(*descs)
+        h = _VP()
+        _check(L.rift_ensemble_create(len(descs), arr, C.byref(h)), "rift_ensemble_create")
+        self.h = h
+        self.members = []
+        for i, geo in enumerate(geos):
+            g = Grid.borrow(L.rift_ensemble_member(self.h, i))
+            g.upload_static(geo["BX"], geo["BY"], geo["n"], geo["K"], geo["h0"])
+            g.init(self._kw[i].get("t0", 0.0))
+            self.members.append(g)
+
+    def set_tf(self, i, tf):
+        _check(load().rift_ensemble_set_tf(self.h, i, float(tf)), "rift_ensemble_set_tf")
+
+    def step(self, n=1):
+        _check(load().rift_ensemble_step(self.h, int(n)), "rift_ensemble_step")
+
+    def finished(self):
+        v = C.c_int()
+        _check(load().rift_ensemble_finished(self.h, C.byref(v)), "rift_ensemble_finished")
+        return v.value
+
+    def run(self, check_every=16, max_steps=-1):
+        n = C.c_long()
+        _check(load().rift_ensemble_run(self.h, int(check_every), int(max_steps), C.byref(n)),
+               "rift_ensemble_run")
+        return n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            for g in self.members:
+                g.close()
+            h, self.h = self.h, None
+            load().rift_ensemble_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Config:
