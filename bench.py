@@ -105,6 +105,47 @@ def cpu_baseline(cfg, g, budget_s=15.0, max_steps=60):
                       f"per-cell update (oracle/liboracle32.so), {el:.1f} s, {threads} threads"}
 
 
+def ensemble_leg(nscen, seed=4, check_every=64):
+    """The batched-ensemble workload (rift_amd.synth.ensemble_scenarios: sides
+    512..2048, dx = 10 m, tf = 1 h, per-scenario rain peak, Manning n and
+    Green-Ampt K) advanced to every scenario's end time by rift_ensemble_run;
+    FP64 volume budgets per scenario.  Scenarios/hour = scenarios / wall time
+    of the run (inputs resident in HBM) x 3600."""
+    from rift_amd import kp, synth
+    t_gen = time.perf_counter()
+    scen = synth.ensemble_scenarios(nscen, seed)
+    flags = kp.TILE_ACCEL | kp.RAIN_AVG | kp.MASS_BUDGET
+    spec = [dict(dem=m["dem"], dx=m["dx"], n_const=m["n_const"], K=m["K"], psi=m["psi"],
+                 dtheta=m["dtheta"], flags=flags) for m in scen]
+    e = kp.Ensemble(spec)
+    for i, (g, m) in enumerate(zip(e.members, scen)):
+        t, v = synth.series_hours_to_seconds(m["hours"], m["mmhr"])
+        g.set_series(kp.SER_HYETO, t, v)
+        e.set_tf(i, m["tf"])
+    cells = sum(m["side"] ** 2 for m in scen)
+    del scen, spec
+    setup_s = time.perf_counter() - t_gen
+    print(f"[bench] ensemble of {nscen} ready ({cells / 1e6:.1f} M cells, {setup_s:.0f} s set-up)",
+          file=sys.stderr, flush=True)
+    e.members[0].sync()
+    t0 = time.perf_counter()
+    steps = e.run(check_every=check_every)
+    e.members[0].sync()
+    wall = time.perf_counter() - t0
+    wcu = sum(g.wet_cells()[1] for g in e.members)
+    worst = 0.0
+    for g in e.members:
+        b = g.budget()
+        worst = max(worst, abs(b[8]) / max(np.abs(b[:8]).max(), 1e-300))
+    out = {"scenarios": nscen, "cells": cells, "ensemble_steps": steps, "wall_s": wall,
+           "scenarios_per_hour": nscen / wall * 3600.0, "wcu_per_s": wcu / wall,
+           "max_budget_residual_rel": worst, "setup_s": setup_s,
+           "workload": "sides {512..2048} dx=10m, tf=1h, lognormal rain peak (median 50 mm/hr), "
+                       "n~U(0.02,0.08), Green-Ampt K lognormal; plain batched solver"}
+    e.close()
+    return out
+
+
 def _active_interior_cells(nx, ny, nbx, act):
     """Interior cells of the tiles in an active list (reference encoding)."""
     total = 0
@@ -145,6 +186,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-size reference "
                          "for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--ensemble", type=int, default=64,
+                    help="scenarios of the batched-ensemble leg (0: skip)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="launch variant of the marching kernels (default: autotuned after the "
                          "warm-up, outside the timed region)")
@@ -245,6 +288,11 @@ def main():
         "sim_time_s": t_sim,
         "mass_balance_m3": {"stored": mb[0], "infiltrated": mb[1], "rain_in": mb[2]},
     }
+    if args.ensemble > 0:
+        print("[bench] ensemble ...", file=sys.stderr, flush=True)
+        line["ensemble"] = ensemble_leg(args.ensemble)
+    else:
+        line["ensemble"] = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(cfg, g)

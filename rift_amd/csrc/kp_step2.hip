@@ -969,13 +969,63 @@ __device__ __forceinline__ int member_units(const Dev& d)
     return min((Stage == 1) ? d.sc->n_units_acc : d.sc->n_units, d.nunits);
 }
 
+// Unit counts of members [c0, c0+64), one member per lane (loads in
+// parallel), with their inclusive prefix sum across the wave.
+template <int Stage>
+__device__ __forceinline__ void member_chunk(const Dev* __restrict__ devs, int nmem, int c0, int lane,
+                                             int& cnt, long long& incl)
+{
+    const int m = c0 + lane;
+    cnt = (m < nmem) ? member_units<Stage>(devs[m]) : 0;
+    incl = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+}
+
+// Locates flat unit index k (k < total): member, base (flat index of the
+// member's first unit) and count.  *chunk / *cbase / cnt / incl cache the
+// 64-member chunk of the last lookup; k never decreases.
+template <int Stage>
+__device__ __forceinline__ void find_member(const Dev* __restrict__ devs, int nmem, int lane, long long k,
+                                            int& c0, long long& cbase, int& cnt, long long& incl,
+                                            int& mem, long long& base, int& mcnt)
+{
+    for (;;) {
+        const long long ctot = __shfl(incl, 63);
+        if (k < cbase + ctot) break;
+        cbase += ctot;
+        c0 += 64;
+        member_chunk<Stage>(devs, nmem, c0, lane, cnt, incl);
+    }
+    const unsigned long long above = __ballot(cbase + incl > k);
+    const int l = (int)__builtin_ctzll(above);
+    mem = c0 + l;
+    mcnt = __shfl(cnt, l);
+    base = cbase + __shfl(incl, l) - mcnt;
+}
+
 template <int Stage, bool Textbook, int Waves, bool Budget>
 __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict__ devs, int nmem, int min_rows)
 {
     const int lane = threadIdx.x & 63;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // total units over all members, chunk by chunk
     long long total = 0;
-    for (int m = 0; m < nmem; ++m) total += (long long)member_units<Stage>(devs[m]) * kTileH;
+    int c0 = 0, cnt = 0;
+    long long incl = 0;
+    for (int c = 0; c < nmem; c += 64) {
+        int cc;
+        long long ic;
+        member_chunk<Stage>(devs, nmem, c, lane, cc, ic);
+        if (c == 0) {
+            cnt = cc;
+            incl = ic;
+        }
+        total += __shfl(ic, 63);
+    }
+    total *= kTileH;
     if (total == 0) return;
     const int G = (int)gridDim.x;
     const int nb = min(G, 8), x = (int)blockIdx.x % nb, q = G / nb, r = G % nb;
@@ -988,20 +1038,15 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
     const long long p0 = total * i0 / nshare, pend = total * i1 / nshare;
     if (p0 >= pend) return;
 
-    // member of the first unit of the share
     long long k = p0 / kTileH;
     int a = (int)(p0 - k * kTileH);
-    int mem = 0;
-    long long base = 0;
-    int cnt = member_units<Stage>(devs[0]);
-    while (k >= base + cnt) {
-        base += cnt;
-        ++mem;
-        cnt = member_units<Stage>(devs[mem]);
-    }
+    long long cbase = 0, base = 0;
+    int mem = 0, mcnt = 0;
+    find_member<Stage>(devs, nmem, lane, k, c0, cbase, cnt, incl, mem, base, mcnt);
     for (;;) {
+        mem = __builtin_amdgcn_readfirstlane(mem);
         const Dev& d = devs[mem];
-        const bool dense = cnt == d.nunits;
+        const bool dense = mcnt == d.nunits;
         const StepScalars S = step_scalars(d);
         // runs of vertically adjacent units of one strip within this member
         const int kl = (int)(k - base);
@@ -1009,7 +1054,7 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
         const int s = u / d.nurows, rr = u - s * d.nurows;
         const int ja = rr * kTileH + 2 + a;
         int last = u, kk = kl;
-        while (kk + 1 < cnt && (base + kk + 1) * kTileH < pend) {
+        while (kk + 1 < mcnt && (base + kk + 1) * kTileH < pend) {
             const int un = __builtin_amdgcn_readfirstlane(dense ? kk + 1 : d.ulist[kk + 1]);
             if (un != last + 1 || un - s * d.nurows >= d.nurows) break;
             last = un;
@@ -1022,11 +1067,7 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
         k = base + kk + 1;
         a = 0;
         if (k * kTileH >= pend) break;
-        while (k >= base + cnt) {   // next member with units
-            base += cnt;
-            ++mem;
-            cnt = member_units<Stage>(devs[mem]);
-        }
+        if (k >= base + mcnt) find_member<Stage>(devs, nmem, lane, k, c0, cbase, cnt, incl, mem, base, mcnt);
     }
 }
 
@@ -1053,11 +1094,14 @@ static void launch_march_ens_w(const Dev* devs, int nmem, unsigned flags, int mi
     else hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, false>), g, b, 0, s, devs, nmem, min_rows);
 }
 
+// Ensembles use stage 1 at 5 waves per SIMD and stage 2 at 4 (3 with the
+// volume budget, whose accumulators would spill at 4).
 void launch_march_ens(const Dev* devs, int nmem, unsigned flags, int stage, int min_rows, int blocks,
                       hipStream_t s)
 {
     blocks = std::max(1, blocks);
     if (stage == 1) launch_march_ens_w<1, 5>(devs, nmem, flags, min_rows, blocks, s);
+    else if (flags & RIFT_MASS_BUDGET) launch_march_ens_w<2, 3>(devs, nmem, flags, min_rows, blocks, s);
     else launch_march_ens_w<2, 4>(devs, nmem, flags, min_rows, blocks, s);
 }
 
@@ -1076,7 +1120,8 @@ static int resident_ens(bool textbook, bool budget, int cus)
 
 int march_ens_resident_blocks(int stage, bool textbook, bool budget, int cus)
 {
-    return stage == 1 ? resident_ens<1, 5>(textbook, budget, cus) : resident_ens<2, 4>(textbook, budget, cus);
+    if (stage == 1) return resident_ens<1, 5>(textbook, budget, cus);
+    return budget ? resident_ens<2, 3>(textbook, budget, cus) : resident_ens<2, 4>(textbook, budget, cus);
 }
 
 }  // namespace rift

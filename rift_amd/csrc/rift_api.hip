@@ -1113,7 +1113,22 @@ struct rift_ensemble {
     int march_blocks[2] = {0, 0};
     int max_threads = 0, max_tiles = 0;
     int min_rows = kTileH;
+    std::vector<Dev> uploaded;      // what devs holds (the graphs read it)
+    hipGraphExec_t graph = nullptr; // kEnsGraphSteps captured steps
 };
+
+constexpr int kEnsGraphSteps = 16;
+
+static void ens_step_launches(rift_ensemble* e)
+{
+    const int n = (int)e->members.size();
+    for (rift_grid* g : e->members)
+        if (g->d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(g->d, e->stream);
+    launch_grow2_ens(e->devs, n, e->max_threads, e->stream);
+    launch_march_ens(e->devs, n, e->flags, 1, e->min_rows, e->march_blocks[0], e->stream);
+    launch_timestep2_ens(e->devs, n, e->max_tiles, e->stream);
+    launch_march_ens(e->devs, n, e->flags, 2, e->min_rows, e->march_blocks[1], e->stream);
+}
 
 extern "C" {
 
@@ -1193,16 +1208,32 @@ int rift_ensemble_step(rift_ensemble* e, int nsteps)
         prime(g);
         e->host_devs[i] = g->d;
     }
-    HIPCHK(hipMemcpyAsync(e->devs, e->host_devs.data(), sizeof(Dev) * n, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));   // host_devs may change before the next call
-    for (int k = 0; k < nsteps; ++k) {
-        for (rift_grid* g : e->members)
-            if (g->d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(g->d, e->stream);
-        launch_grow2_ens(e->devs, n, e->max_threads, e->stream);
-        launch_march_ens(e->devs, n, e->flags, 1, e->min_rows, e->march_blocks[0], e->stream);
-        launch_timestep2_ens(e->devs, n, e->max_tiles, e->stream);
-        launch_march_ens(e->devs, n, e->flags, 2, e->min_rows, e->march_blocks[1], e->stream);
+    const bool same = e->uploaded.size() == (size_t)n &&
+                      std::memcmp(e->uploaded.data(), e->host_devs.data(), sizeof(Dev) * n) == 0;
+    if (!same) {
+        if (e->graph) {
+            (void)hipGraphExecDestroy(e->graph);
+            e->graph = nullptr;
+        }
+        HIPCHK(hipMemcpyAsync(e->devs, e->host_devs.data(), sizeof(Dev) * n, hipMemcpyHostToDevice,
+                              e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        e->uploaded = e->host_devs;
     }
+    int left = nsteps;
+    if (left >= kEnsGraphSteps && !e->graph) {
+        // the ensemble's step loop as one graph (no memset nodes)
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < kEnsGraphSteps; ++k) ens_step_launches(e);
+        hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+        if (ce != hipSuccess) return fail(RIFT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        ce = hipGraphInstantiate(&e->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ce != hipSuccess) return fail(RIFT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ce));
+    }
+    for (; left >= kEnsGraphSteps; left -= kEnsGraphSteps) HIPCHK(hipGraphLaunch(e->graph, e->stream));
+    for (; left > 0; --left) ens_step_launches(e);
     for (rift_grid* g : e->members) g->scattered = true;
     return check_launch("rift_ensemble_step");
 }
@@ -1243,6 +1274,7 @@ int rift_ensemble_destroy(rift_ensemble* e)
 {
     if (!e) return RIFT_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->graph) (void)hipGraphExecDestroy(e->graph);
     for (rift_grid* g : e->members) rift_destroy(g);
     if (e->devs) (void)hipFree(e->devs);
     if (e->stream) (void)hipStreamDestroy(e->stream);

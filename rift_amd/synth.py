@@ -222,3 +222,31 @@ def write_case(case, outdir, name="case", **keywords):
         for k, v in cfg.items():
             f.write(f"{k} = {v!r}\n" if isinstance(v, float) else f"{k} = {v}\n")
     return path
+
+
+ENSEMBLE_SIDES = (512, 768, 1024, 1536, 2048)
+
+
+def ensemble_scenarios(n=64, seed=4, sides=ENSEMBLE_SIDES, tf=3600.0):
+    """The batched-ensemble workload: n scenarios with the side drawn
+    uniformly from `sides` (dx = 10 m), a rain_basin-style fractal DEM
+    (0-60 m, 3 m channel) per scenario, Manning n ~ U(0.02, 0.08), Green-Ampt
+    K lognormal (median 5 mm/hr, sigma_ln 0.8), psi 0.11, dtheta 0.3, and a
+    triangular hyetograph over 1 h whose peak is lognormal (median 50 mm/hr,
+    sigma_ln 0.5).  Returns member dicts (the Ensemble / grid_from_rasters
+    arguments plus `hours`, `mmhr`, `tf`)."""
+    u = uniform(seed, 3 * n, 11)
+    g = normal(seed, n, 12)
+    out = []
+    for k in range(n):
+        side = sides[min(int(u[k] * len(sides)), len(sides) - 1)]
+        dem = 60.0 * fractal_dem(side, 0.6, seed * 1000 + k)
+        dem = carved_channel(dem, 3.0, seed * 1000 + k)
+        N = side + 1
+        K = 5.0 * np.exp(0.8 * normal(seed * 1000 + k, N * N, 13).reshape(N, N))
+        n_man = 0.02 + 0.06 * u[n + k]
+        peak = 50.0 * np.exp(0.5 * g[k])
+        out.append({"dem": dem, "dx": 10.0, "n_const": float(n_man), "K": K, "psi": 0.11,
+                    "dtheta": 0.3, "hours": np.array([0.0, 0.5, 1.0], np.float32),
+                    "mmhr": np.array([0.0, peak, 0.0], np.float32), "tf": tf, "side": side})
+    return out
