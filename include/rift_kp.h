@@ -198,6 +198,22 @@ int rift_set_time(rift_grid* g, double t, double dt);
 int rift_download(rift_grid* g, int field, double* host);
 int rift_upload(rift_grid* g, int field, const double* host);
 
+/* Device-resident field windows (no reference counterpart: they let a
+ * training or screening pipeline collect solver maps without a host round
+ * trip).  Copies the FP32 window [i0, i0+w) x [j0, j0+h) of a stored field
+ * (RIFT_DEPTH, RIFT_HU, ..., RIFT_HMAX, RIFT_TWET, ...; not the derived
+ * RIFT_W, RIFT_Q, RIFT_BC) to device memory dst, row-major h x w floats,
+ * ordered on the handle's stream; rift_sync waits for it. */
+int rift_copy_field(rift_grid* g, int field, int i0, int j0, int w, int h, float* dst);
+
+/* Device memory owned by the caller (no reference counterpart), for the
+ * dst of rift_copy_field and the arrays of rift_unet.h.  rift_memcpy is
+ * synchronous; kind 0 = host to device, 1 = device to host, 2 = device to
+ * device. */
+int rift_device_alloc(int device, size_t bytes, void** ptr);
+int rift_device_free(void* ptr);
+int rift_memcpy(void* dst, const void* src, size_t bytes, int kind);
+
 /* Tile state: wet flags and active flags (ntiles bytes each) and the active
  * count, replacing the wet_blocks/active_blocks copies of the hotstart. */
 int rift_get_tiles(rift_grid* g, unsigned char* wet, unsigned char* active, int* n_active);

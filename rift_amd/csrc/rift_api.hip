@@ -892,6 +892,47 @@ int rift_download(rift_grid* g, int field, double* host)
     return RIFT_OK;
 }
 
+int rift_copy_field(rift_grid* g, int field, int i0, int j0, int w, int h, float* dst)
+{
+    if (!g || !dst) return fail(RIFT_E_ARG, "rift_copy_field: null argument");
+    {
+        int rc0 = ensure_canonical(g);
+        if (rc0) return rc0;
+    }
+    Dev& d = g->d;
+    if (i0 < 0 || j0 < 0 || w < 1 || h < 1 || i0 + w > d.nx || j0 + h > d.ny)
+        return fail(RIFT_E_ARG, "rift_copy_field: window outside the grid");
+    float* p = field_plane(g, field);
+    if (!p) return fail(RIFT_E_ARG, "rift_copy_field: field not stored with these flags");
+    HIPCHK(hipSetDevice(g->desc.device));
+    HIPCHK(hipMemcpy2DAsync(dst, (size_t)w * sizeof(float), p + (size_t)j0 * d.P + i0, (size_t)d.P * sizeof(float),
+                            (size_t)w * sizeof(float), (size_t)h, hipMemcpyDeviceToDevice, g->stream));
+    return RIFT_OK;
+}
+
+int rift_device_alloc(int device, size_t bytes, void** ptr)
+{
+    if (!ptr || bytes == 0) return fail(RIFT_E_ARG, "rift_device_alloc: bad argument");
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMalloc(ptr, bytes));
+    return RIFT_OK;
+}
+
+int rift_device_free(void* ptr)
+{
+    if (ptr) HIPCHK(hipFree(ptr));
+    return RIFT_OK;
+}
+
+int rift_memcpy(void* dst, const void* src, size_t bytes, int kind)
+{
+    if ((!dst || !src) && bytes) return fail(RIFT_E_ARG, "rift_memcpy: null argument");
+    if (kind < 0 || kind > 2) return fail(RIFT_E_ARG, "rift_memcpy: kind must be 0, 1 or 2");
+    const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    if (bytes) HIPCHK(hipMemcpy(dst, src, bytes, k[kind]));
+    return RIFT_OK;
+}
+
 int rift_upload(rift_grid* g, int field, const double* host)
 {
     if (!g || !host) return fail(RIFT_E_ARG, "rift_upload: null argument");

@@ -55,6 +55,12 @@ class RiftDesc(C.Structure):
                 ("device", C.c_int)]
 
 
+class UNetDesc(C.Structure):
+    _fields_ = [("height", C.c_int), ("width", C.c_int), ("in_channels", C.c_int),
+                ("out_channels", C.c_int), ("base_channels", C.c_int), ("levels", C.c_int),
+                ("max_batch", C.c_int), ("seed", C.c_ulonglong), ("device", C.c_int)]
+
+
 SIGNATURES = [
     ("rift_last_error", C.c_char_p, []),
     ("rift_create", C.c_int, [C.POINTER(RiftDesc), C.POINTER(_VP)]),
@@ -126,6 +132,21 @@ SIGNATURES = [
     ("rift_sim_write_hotstart", C.c_int, [_VP, C.c_char_p]),
     ("rift_sim_read_hotstart", C.c_int, [_VP, C.c_char_p]),
     ("rift_sim_close", C.c_int, [_VP]),
+    ("rift_copy_field", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
+    ("rift_device_alloc", C.c_int, [C.c_int, C.c_size_t, C.POINTER(_VP)]),
+    ("rift_device_free", C.c_int, [_VP]),
+    ("rift_memcpy", C.c_int, [_VP, _VP, C.c_size_t, C.c_int]),
+    ("rift_unet_create", C.c_int, [C.POINTER(UNetDesc), C.POINTER(_VP)]),
+    ("rift_unet_destroy", C.c_int, [_VP]),
+    ("rift_unet_param_count", C.c_int, [_VP, C.POINTER(C.c_long)]),
+    ("rift_unet_get_params", C.c_int, [_VP, _F]),
+    ("rift_unet_set_params", C.c_int, [_VP, _F]),
+    ("rift_unet_set_loss_weights", C.c_int, [_VP, _F]),
+    ("rift_unet_set_adam", C.c_int, [_VP, C.c_float, C.c_float, C.c_float]),
+    ("rift_unet_forward", C.c_int, [_VP, _VP, C.c_int, _VP]),
+    ("rift_unet_gradient", C.c_int, [_VP, _VP, _VP, C.c_int, _D, _F]),
+    ("rift_unet_train_step", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_float, _D]),
+    ("rift_unet_flops", C.c_int, [_VP, C.c_int, _D, _D]),
 ]
 
 _lib = None
@@ -180,6 +201,57 @@ def geometry(dem, h0=None, n=None, n_const=0.0, K=None):
                                 _dp(arrs[2]), _dp(BX), _dp(BY), _dp(H0), _dp(N), _dp(KK)),
            "rift_geometry")
     return {"BX": BX, "BY": BY, "h0": H0, "n": N, "K": KK, "nx": nx, "ny": ny}
+
+
+class DeviceArray:
+    """float32 device memory allocated through the library
+    (rift_device_alloc); `ptr(i)` is the address of flat element i."""
+
+    def __init__(self, shape, device=0):
+        self.shape = tuple(int(v) for v in np.atleast_1d(shape))
+        self.size = int(np.prod(self.shape))
+        self.device = device
+        p = _VP()
+        _check(load().rift_device_alloc(device, 4 * max(self.size, 1), C.byref(p)), "rift_device_alloc")
+        self._p = p.value
+
+    def ptr(self, i=0):
+        if not 0 <= i <= self.size:
+            raise RiftError("DeviceArray.ptr: index out of range")
+        return self._p + 4 * int(i)
+
+    def upload(self, a, offset=0):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if offset < 0 or offset + a.size > self.size:
+            raise RiftError("DeviceArray.upload: out of range")
+        _check(load().rift_memcpy(C.c_void_p(self.ptr(offset)), a.ctypes.data_as(_VP), 4 * a.size, 0),
+               "rift_memcpy")
+
+    def download(self, count=None, offset=0, shape=None):
+        count = self.size - offset if count is None else int(count)
+        if offset < 0 or count < 0 or offset + count > self.size:
+            raise RiftError("DeviceArray.download: out of range")
+        out = np.empty(count, np.float32)
+        _check(load().rift_memcpy(out.ctypes.data_as(_VP), C.c_void_p(self.ptr(offset)), 4 * count, 1),
+               "rift_memcpy")
+        return out.reshape(shape if shape is not None else ((count,) if count != self.size else self.shape))
+
+    def copy_from(self, other, count, dst_offset=0, src_offset=0):
+        if dst_offset + count > self.size or src_offset + count > other.size:
+            raise RiftError("DeviceArray.copy_from: out of range")
+        _check(load().rift_memcpy(C.c_void_p(self.ptr(dst_offset)), C.c_void_p(other.ptr(src_offset)),
+                                  4 * int(count), 2), "rift_memcpy")
+
+    def free(self):
+        if getattr(self, "_p", None):
+            load().rift_device_free(C.c_void_p(self._p))
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Grid:
@@ -306,6 +378,12 @@ class Grid:
         out = np.empty((self.ny, self.nx))
         _check(load().rift_download(self.h, FIELDS.index(name), _dp(out)), "rift_download " + name)
         return out
+
+    def copy_field(self, name, dst, i0, j0, w, h):
+        """Window [i0, i0+w) x [j0, j0+h) of a stored field into device memory
+        at pointer `dst` (h x w float32), ordered on the grid's stream."""
+        _check(load().rift_copy_field(self.h, FIELDS.index(name), i0, j0, w, h, C.c_void_p(int(dst))),
+               "rift_copy_field " + name)
 
     def set_field(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64)
