@@ -146,6 +146,29 @@ def ensemble_leg(nscen, seed=4, check_every=64):
     return out
 
 
+def surrogate_leg(data, epochs):
+    """Surrogate screening (rift_amd.surrogate.run) at reduced size: the
+    network of the full workload (4 levels, 32 base channels, BF16 on the
+    matrix cores) trained on len(train) solver runs at 256 x 256 (dx = 10 m,
+    tf = 1 h), then screening len(test) new scenarios with solver fallback
+    within 0.1 m of a 0.3 m asset-depth threshold and 5 % held out.  The full
+    workload (4000 training runs, 2048 test scenarios) is
+    `python -m rift_amd.surrogate`."""
+    from rift_amd import surrogate
+    r = surrogate.run(epochs=epochs, data=data,
+                      log=lambda m: print("[bench] surrogate: " + m, file=sys.stderr, flush=True))
+    keep = ["n_train", "n_test", "epochs", "train_solver_s", "train_s", "train_tflops", "infer_s",
+            "infer_tflops", "fallback", "heldout", "solver_runs", "screened_per_hour", "plain_per_hour",
+            "gain", "gain_without_heldout_est", "heldout_errors", "all_errors_surrogate",
+            "all_errors_screened", "decision_accuracy", "flooded_frac"]
+    out = {k: r.get(k) for k in keep}
+    out["final_train_loss"] = r["train_loss"][-1] if r["train_loss"] else None
+    out["workload"] = (f"U-Net 4 levels x 32 base ch, BF16 MFMA; {r['n_train']} training runs / "
+                       f"{r['n_test']} test scenarios at 256x256 (reduced from 4000 / 2048), "
+                       "threshold 0.3 m +- 0.1 m at the central 8x8 cells, 5% held out")
+    return out
+
+
 def _active_interior_cells(nx, ny, nbx, act):
     """Interior cells of the tiles in an active list (reference encoding)."""
     total = 0
@@ -191,12 +214,27 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="launch variant of the marching kernels (default: autotuned after the "
                          "warm-up, outside the timed region)")
+    ap.add_argument("--surrogate-train", type=int, default=512,
+                    help="training scenarios of the surrogate-screening leg (0: skip the leg)")
+    ap.add_argument("--surrogate-test", type=int, default=512,
+                    help="test scenarios of the surrogate-screening leg")
+    ap.add_argument("--surrogate-epochs", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the solver runs one domain per GPU; see DESIGN.md for how it would shard)",
               file=sys.stderr)
         sys.exit(2)
+
+    surr_data = None
+    if args.surrogate_train > 0:
+        # scenario inputs come from a process pool, before this process
+        # touches the GPU
+        from rift_amd import surrogate
+        t0 = time.perf_counter()
+        surr_data = surrogate.prepare(args.surrogate_train, args.surrogate_test, workers=8)
+        print(f"[bench] surrogate scenarios generated ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
+              flush=True)
 
     g, cfg = build_workload(args.cells)
     print(f"[bench] workload ready: {g.nx}x{g.ny} internal, {g.ntiles} tiles", file=sys.stderr,
@@ -293,6 +331,12 @@ def main():
         line["ensemble"] = ensemble_leg(args.ensemble)
     else:
         line["ensemble"] = None
+    if surr_data is not None:
+        print("[bench] surrogate screening ...", file=sys.stderr, flush=True)
+        line["surrogate"] = surrogate_leg(surr_data, args.surrogate_epochs)
+        del surr_data
+    else:
+        line["surrogate"] = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(cfg, g)

@@ -126,3 +126,334 @@ class UNet:
             self.close()
         except Exception:
             pass
+
+
+# --- the screening workflow -------------------------------------------------
+
+SIDE = 256              # cells per side of a surrogate scenario (dx = 10 m)
+TF = 3600.0             # simulated time, s
+ASSET = 8               # decision window: the ASSET x ASSET cells at the centre
+THRESHOLD = 0.3         # m of maximum depth at the asset
+MARGIN = 0.1            # predictions within THRESHOLD +- MARGIN go to the solver
+FEATURES = ("elevation", "relief_9", "relief_33", "slope_x", "slope_y", "manning_n", "rain_peak", "log_K")
+TARGETS = ("max_depth_m", "arrival_h")
+SOLVER_FLAGS = kp.TILE_ACCEL | kp.RAIN_AVG | kp.SAVE_MAX | kp.SAVE_ARRIVAL
+
+
+def scenario(seed, side=SIDE, tf=TF):
+    """One seeded scenario in the style of the batched ensemble: fractal DEM
+    (0-60 m) with a 3 m channel, Manning n ~ U(0.02, 0.08), triangular
+    hyetograph over 1 h with a lognormal peak (median 50 mm/hr, sigma_ln 0.5),
+    Green-Ampt K lognormal (median 5 mm/hr, sigma_ln 0.8), psi 0.11,
+    dtheta 0.3."""
+    from . import synth
+    u = synth.uniform(seed, 1, 21)[0]
+    z = synth.normal(seed, 1, 22)[0]
+    dem = synth.carved_channel(60.0 * synth.fractal_dem(side, 0.6, seed), 3.0, seed)
+    N = side + 1
+    K = 5.0 * np.exp(0.8 * synth.normal(seed, N * N, 13).reshape(N, N))
+    return {"seed": seed, "dem": dem, "dx": 10.0, "n_const": float(0.02 + 0.06 * u), "K": K, "psi": 0.11,
+            "dtheta": 0.3, "hours": np.array([0.0, 0.5, 1.0], np.float32),
+            "mmhr": np.array([0.0, 50.0 * np.exp(0.5 * z), 0.0], np.float32), "tf": tf, "side": side}
+
+
+def _box_mean(a, w):
+    """Mean over a w x w window (edge-replicated) by summed-area table."""
+    r = w // 2
+    p = np.pad(a, r, mode="edge")
+    c = np.zeros((p.shape[0] + 1, p.shape[1] + 1))
+    c[1:, 1:] = p.cumsum(0).cumsum(1)
+    n0, n1 = a.shape
+    return (c[w:w + n0, w:w + n1] - c[:n0, w:w + n1] - c[w:w + n0, :n1] + c[:n0, :n1]) / (w * w)
+
+
+def features(sc):
+    """Network input (8, side, side) float32 at cell centres (the mean of a
+    cell's four corner points, as the solver forms cell values).  The
+    network computes in bfloat16 (8 significant bits), so elevation enters
+    at the scales flow depends on rather than only as an absolute level:
+    elevation above the lowest cell / 60 m; relief against the 9 x 9 and
+    33 x 33 cell means / 2 m and / 5 m; central-difference slopes / 0.05;
+    (n - 0.05) / 0.02; ln(peak / 50 mm/hr) / 0.5; ln(K / 5 mm/hr) / 0.8."""
+    def centre(a):
+        return 0.25 * (a[:-1, :-1] + a[1:, :-1] + a[:-1, 1:] + a[1:, 1:])
+    z = centre(sc["dem"])
+    dx = sc["dx"]
+    zp = np.pad(z, 1, mode="edge")
+    out = np.empty((len(FEATURES),) + z.shape, np.float32)
+    out[0] = (z - z.min()) / 60.0
+    out[1] = (z - _box_mean(z, 9)) / 2.0
+    out[2] = (z - _box_mean(z, 33)) / 5.0
+    out[3] = (zp[1:-1, 2:] - zp[1:-1, :-2]) / (2.0 * dx) / 0.05
+    out[4] = (zp[2:, 1:-1] - zp[:-2, 1:-1]) / (2.0 * dx) / 0.05
+    out[5] = (sc["n_const"] - 0.05) / 0.02
+    out[6] = np.log(float(sc["mmhr"].max()) / 50.0) / 0.5
+    out[7] = centre(np.log(sc["K"] / 5.0) / 0.8)
+    return out
+
+
+def _gen(args):
+    seed, side, tf = args
+    sc = scenario(seed, side, tf)
+    return sc, features(sc)
+
+
+def generate(seeds, side=SIDE, tf=TF, workers=1):
+    """(scenarios, features) for the seeds; `workers` > 1 uses a process pool
+    (create it before the process touches the GPU)."""
+    args = [(int(s), side, tf) for s in seeds]
+    if workers > 1:
+        import multiprocessing as mp
+        with mp.get_context("fork").Pool(workers) as pool:
+            res = pool.map(_gen, args, chunksize=8)
+    else:
+        res = [_gen(a) for a in args]
+    return [r[0] for r in res], np.stack([r[1] for r in res])
+
+
+def solve(scs, chunk=256, device=0, log=None):
+    """The plain batched solver on the scenarios, `chunk` members per
+    ensemble.  Returns labels (n, 2, side, side) float32 -- maximum depth (m)
+    and arrival time (h; never wet = tf) -- and timings: setup_s (grid
+    creation, geometry, uploads), run_s (the stepping), copy_s."""
+    import time
+    n = len(scs)
+    side = scs[0]["side"]
+    hw = side * side
+    labels = np.empty((n, 2, side, side), np.float32)
+    stats = {"setup_s": 0.0, "run_s": 0.0, "copy_s": 0.0, "steps": 0}
+    for c0 in range(0, n, chunk):
+        part = scs[c0:c0 + chunk]
+        t0 = time.perf_counter()
+        spec = [dict(dem=s["dem"], dx=s["dx"], n_const=s["n_const"], K=s["K"], psi=s["psi"],
+                     dtheta=s["dtheta"], flags=SOLVER_FLAGS, device=device) for s in part]
+        e = kp.Ensemble(spec)
+        from . import synth
+        for i, (g, s) in enumerate(zip(e.members, part)):
+            t, v = synth.series_hours_to_seconds(s["hours"], s["mmhr"])
+            g.set_series(kp.SER_HYETO, t, v)
+            e.set_tf(i, s["tf"])
+        e.members[0].sync()
+        t1 = time.perf_counter()
+        stats["steps"] = max(stats["steps"], e.run(check_every=64))
+        e.members[0].sync()
+        t2 = time.perf_counter()
+        lab = kp.DeviceArray((len(part), 2, side, side), device)
+        for i, g in enumerate(e.members):
+            g.copy_field("h_max", lab.ptr(2 * i * hw), 2, 2, side, side)
+            g.copy_field("t_wet", lab.ptr((2 * i + 1) * hw), 2, 2, side, side)
+        e.members[0].sync()
+        arr = lab.download()
+        lab.free()
+        e.close()
+        t3 = time.perf_counter()
+        tf = np.array([s["tf"] for s in part], np.float32)[:, None, None]
+        arr[:, 1] = np.where(arr[:, 1] < 0.0, tf, arr[:, 1]) / 3600.0
+        labels[c0:c0 + len(part)] = arr
+        stats["setup_s"] += t1 - t0
+        stats["run_s"] += t2 - t1
+        stats["copy_s"] += t3 - t2
+        if log:
+            log(f"solved {c0 + len(part)}/{n} (setup {t1 - t0:.1f} s, run {t2 - t1:.1f} s)")
+    return labels, stats
+
+
+def asset_depth(maps):
+    """Decision quantity per scenario: the largest maximum depth over the
+    ASSET x ASSET cells at the centre of the domain.  maps (n, side, side)."""
+    side = maps.shape[-1]
+    a = side // 2 - ASSET // 2
+    return maps[:, a:a + ASSET, a:a + ASSET].reshape(len(maps), -1).max(axis=1)
+
+
+def train(net, X, Y, epochs, batch, lr, seed=0, log=None, lr_end_frac=0.05):
+    """Adam over the (n, 4, H, W) inputs X and (n, 2, H, W) labels Y held in
+    device memory, `epochs` passes in seeded random order, cosine learning
+    rate from lr to lr * lr_end_frac.  Returns per-epoch mean losses and the
+    training seconds."""
+    import time
+    n = X.shape[0]
+    sx = int(np.prod(X.shape[1:]))   # floats per sample
+    sy = int(np.prod(Y.shape[1:]))
+    dX = kp.DeviceArray(X.shape, net.desc.device)
+    dY = kp.DeviceArray(Y.shape, net.desc.device)
+    dX.upload(X)
+    dY.upload(Y)
+    bX = kp.DeviceArray((batch,) + X.shape[1:], net.desc.device)
+    bY = kp.DeviceArray((batch,) + Y.shape[1:], net.desc.device)
+    rng = np.random.default_rng(seed)
+    steps_per_epoch = n // batch
+    total = max(1, epochs * steps_per_epoch)
+    hist = []
+    step = 0
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        acc = 0.0
+        for s in range(steps_per_epoch):
+            idx = order[s * batch:(s + 1) * batch]
+            for j, k in enumerate(idx):
+                bX.copy_from(dX, sx, j * sx, int(k) * sx)
+                bY.copy_from(dY, sy, j * sy, int(k) * sy)
+            f = step / total
+            cur = lr * (lr_end_frac + (1.0 - lr_end_frac) * 0.5 * (1.0 + np.cos(np.pi * f)))
+            acc += net.train_step(bX, bY, batch, cur)
+            step += 1
+        hist.append(acc / max(steps_per_epoch, 1))
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.5f} ({time.perf_counter() - t0:.1f} s)")
+    seconds = time.perf_counter() - t0
+    for a in (dX, dY, bX, bY):
+        a.free()
+    return hist, seconds
+
+
+def predict(net, X, batch=None):
+    """Network outputs for inputs X (n, 4, H, W) (host), in batches; returns
+    (n, 2, H, W) float32 and the seconds spent (uploads, network, downloads)."""
+    import time
+    batch = batch or net.max_batch
+    n = X.shape[0]
+    out = np.empty((n, net.out_channels, net.height, net.width), np.float32)
+    bX = kp.DeviceArray((batch,) + X.shape[1:], net.desc.device)
+    bY = kp.DeviceArray((batch,) + out.shape[1:], net.desc.device)
+    t0 = time.perf_counter()
+    for s in range(0, n, batch):
+        m = min(batch, n - s)
+        bX.upload(X[s:s + m])
+        net.forward(bX, m, bY)
+        out[s:s + m] = bY.download(m * int(np.prod(out.shape[1:]))).reshape((m,) + out.shape[1:])
+    seconds = time.perf_counter() - t0
+    bX.free()
+    bY.free()
+    return out, seconds
+
+
+def error_stats(pred, truth):
+    """Maximum-depth and arrival errors of predictions against solver maps
+    (n, 2, H, W): mean absolute error, RMSE, the 95th percentile of the
+    per-scenario mean absolute error, and the asset-depth error."""
+    dh = np.abs(pred[:, 0] - truth[:, 0])
+    per = dh.reshape(len(dh), -1).mean(axis=1)
+    qa, qt = asset_depth(pred[:, 0]), asset_depth(truth[:, 0])
+    return {"depth_mae_m": float(dh.mean()), "depth_rmse_m": float(np.sqrt((dh ** 2).mean())),
+            "depth_mae_p95_m": float(np.percentile(per, 95)),
+            "asset_depth_mae_m": float(np.abs(qa - qt).mean()),
+            "arrival_mae_h": float(np.abs(pred[:, 1] - truth[:, 1]).mean())}
+
+
+def screen(net, scs, X, heldout_frac=0.05, seed=1, chunk=256, log=None):
+    """Surrogate screening of the scenarios (features X): predict every
+    scenario, send to the solver those whose predicted asset depth lies
+    within MARGIN of THRESHOLD plus a seeded held-out share, and judge the
+    surrogate's error on the held-out scenarios.  Returns the decisions,
+    the final maps (solver maps where the solver ran) and the timings."""
+    pred, t_inf = predict(net, X)
+    q = asset_depth(pred[:, 0])
+    near = np.abs(q - THRESHOLD) < MARGIN
+    rng = np.random.default_rng(seed)
+    held = np.zeros(len(scs), bool)
+    held[rng.choice(len(scs), max(1, int(round(heldout_frac * len(scs)))), replace=False)] = True
+    run = np.flatnonzero(near | held)
+    truth, st = solve([scs[i] for i in run], chunk=chunk, device=net.desc.device, log=log)
+    final = pred.copy()
+    final[run] = truth
+    pos = {i: k for k, i in enumerate(run)}
+    hk = np.array([pos[i] for i in np.flatnonzero(held)])
+    err = error_stats(pred[held], truth[hk])
+    decided = np.where(near, asset_depth(final[:, 0]) > THRESHOLD, q > THRESHOLD)
+    return {"pred": pred, "final": final, "flooded": decided, "fallback": near, "heldout": held,
+            "solver_runs": run, "truth": truth, "infer_s": t_inf, "solver": st, "errors": err}
+
+
+def prepare(n_train, n_test, train_seed0=5000, test_seed0=9000, workers=16, side=SIDE, tf=TF):
+    """Training and test scenarios with their features (host only)."""
+    scs_tr, Xtr = generate(range(train_seed0, train_seed0 + n_train), side, tf, workers)
+    scs_te, Xte = generate(range(test_seed0, test_seed0 + n_test), side, tf, workers)
+    return scs_tr, Xtr, scs_te, Xte
+
+
+def run(n_train=4000, n_test=2048, epochs=20, batch=16, lr=1e-3, train_seed0=5000, test_seed0=9000,
+        heldout=0.05, workers=16, chunk=256, solve_all=True, log=None, device=0, data=None):
+    """The surrogate-screening workload: solver runs for n_train training
+    scenarios (seeds train_seed0 + k), training, screening of n_test new
+    scenarios (seeds test_seed0 + k) with solver fallback and a held-out
+    share, and -- with solve_all -- the plain batched solver over every test
+    scenario as the baseline rate and for the error over all of them.
+    Scenario inputs are generated on the host before the timed parts and are
+    not counted in either rate; `data` = prepare(...) passes them in when they
+    were generated earlier (before the process touched the GPU)."""
+    import time
+    log = log or (lambda m: None)
+    t0 = time.perf_counter()
+    if data is None:
+        data = prepare(n_train, n_test, train_seed0, test_seed0, workers)
+    scs_tr, Xtr, scs_te, Xte = data
+    n_train, n_test = len(scs_tr), len(scs_te)
+    del data
+    gen_s = time.perf_counter() - t0
+    log(f"generated {n_train} + {n_test} scenarios in {gen_s:.1f} s")
+    Ytr, st_tr = solve(scs_tr, chunk=chunk, device=device, log=log)
+    del scs_tr
+    side = scs_te[0]["side"]
+    net = UNet(side, side, len(FEATURES), 2, 32, 4, max_batch=batch, seed=1, device=device)
+    net.set_loss_weights([1.0, 0.25])
+    fwd_flops, train_flops = net.flops(batch)
+    hist, train_s = train(net, Xtr, Ytr, epochs, batch, lr, seed=2, log=log)
+    del Xtr, Ytr
+    res = screen(net, scs_te, Xte, heldout_frac=heldout, seed=3, chunk=chunk, log=log)
+    st = res["solver"]
+    runs = len(res["solver_runs"])
+    screened_s = res["infer_s"] + st["setup_s"] + st["run_s"] + st["copy_s"]
+    out = {
+        "n_train": n_train, "n_test": n_test, "epochs": epochs, "batch": batch, "lr": lr,
+        "side": side, "threshold_m": THRESHOLD, "margin_m": MARGIN, "asset_cells": ASSET * ASSET,
+        "heldout_frac": heldout, "generate_s": gen_s,
+        "train_solver_s": st_tr["setup_s"] + st_tr["run_s"] + st_tr["copy_s"],
+        "train_s": train_s, "train_loss": hist,
+        "train_tflops": train_flops * epochs * (n_train // batch) / max(train_s, 1e-9) / 1e12,
+        "infer_s": res["infer_s"], "infer_tflops": fwd_flops / batch * n_test / max(res["infer_s"], 1e-9) / 1e12,
+        "fallback": int(res["fallback"].sum()), "heldout": int(res["heldout"].sum()), "solver_runs": runs,
+        "screened_s": screened_s, "screened_per_hour": n_test / screened_s * 3600.0,
+        "heldout_errors": res["errors"],
+    }
+    if solve_all:
+        Yte, st_all = solve(scs_te, chunk=chunk, device=device, log=log)
+        plain_s = st_all["setup_s"] + st_all["run_s"] + st_all["copy_s"]
+        truth_flood = asset_depth(Yte[:, 0]) > THRESHOLD
+        out.update({
+            "plain_s": plain_s, "plain_per_hour": n_test / plain_s * 3600.0,
+            "gain": (n_test / screened_s) / (n_test / plain_s),
+            "all_errors_surrogate": error_stats(res["pred"], Yte),
+            "all_errors_screened": error_stats(res["final"], Yte),
+            "decision_accuracy": float(np.mean(res["flooded"] == truth_flood)),
+            "flooded_frac": float(truth_flood.mean()),
+        })
+        # the fallback runs alone (held-out runs as a validation cost apart)
+        fb_only = int((res["fallback"] & ~res["heldout"]).sum())
+        est = res["infer_s"] + (st["setup_s"] + st["run_s"] + st["copy_s"]) * fb_only / max(runs, 1)
+        out["gain_without_heldout_est"] = (n_test / est) / (n_test / plain_s)
+    net.close()
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="Surrogate-screening workload on one GPU")
+    ap.add_argument("--train", type=int, default=4000)
+    ap.add_argument("--test", type=int, default=2048)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    r = run(a.train, a.test, a.epochs, a.batch, a.lr, workers=a.workers,
+            log=lambda m: print("[surrogate] " + m, file=sys.stderr, flush=True))
+    txt = json.dumps(r)
+    print(txt)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(txt + "\n")

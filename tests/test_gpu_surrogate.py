@@ -1,0 +1,84 @@
+"""The surrogate-screening workflow on the GPU at small size (64 x 64
+scenarios): solver labels through device field copies, training through the
+library's kernels, and screening with solver fallback.
+
+What the trained network gains over the method without it is measured in
+scenarios per hour against the plain batched solver on the same scenarios;
+here it must exceed 1 with the network's error on the test scenarios below
+that of the untrained network.  The full-size figures (4000 training runs,
+2048 test scenarios at 256 x 256) come from `python -m rift_amd.surrogate`.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIDE, TF = 64, 1800.0
+
+
+@pytest.fixture(scope="module")
+def S(rift):
+    from rift_amd import surrogate
+    return surrogate
+
+
+def test_solver_labels_match_single_domain_runs(S, rift):
+    """Labels from batched solver runs (two ensembles) equal the h_max and
+    t_wet fields of each scenario stepped alone to its end time."""
+    kp = rift
+    from rift_amd import synth
+    scs = [S.scenario(7000 + k, side=SIDE, tf=600.0) for k in range(3)]
+    Y, st = S.solve(scs, chunk=2)
+    assert Y.shape == (3, 2, SIDE, SIDE) and st["steps"] > 0
+    for k, sc in enumerate(scs):
+        print("scenario", k, flush=True)
+        g = kp.grid_from_rasters(sc["dem"], sc["dx"], n_const=sc["n_const"], K=sc["K"], psi=sc["psi"],
+                                 dtheta=sc["dtheta"], flags=S.SOLVER_FLAGS)
+        t, v = synth.series_hours_to_seconds(sc["hours"], sc["mmhr"])
+        g.set_series(kp.SER_HYETO, t, v)
+        while g.time[0] < sc["tf"]:
+            g.step(1)
+        h = g.field("h_max")[2:-2, 2:-2].astype(np.float32)
+        tw = g.field("t_wet")[2:-2, 2:-2].astype(np.float32)
+        tw = np.where(tw < 0, np.float32(sc["tf"]), tw) / np.float32(3600.0)
+        np.testing.assert_array_equal(Y[k, 0], h)
+        np.testing.assert_array_equal(Y[k, 1], tw)
+        assert h.max() > 0
+        g.close()
+
+
+def test_training_and_screening(S):
+    scs_tr, Xtr = S.generate(range(5000, 5096), side=SIDE, tf=TF)
+    scs_te, Xte = S.generate(range(9000, 9048), side=SIDE, tf=TF)
+    Ytr, _ = S.solve(scs_tr, chunk=96)
+    Yte, st = S.solve(scs_te, chunk=48)
+    plain_s = st["setup_s"] + st["run_s"] + st["copy_s"]
+    net = S.UNet(SIDE, SIDE, len(S.FEATURES), 2, 32, 4, max_batch=8, seed=1)
+    net.set_loss_weights([1.0, 0.25])
+    P0, _ = S.predict(net, Xte)
+    e0 = S.error_stats(P0, Yte)
+    hist, _ = S.train(net, Xtr, Ytr, epochs=10, batch=8, lr=2e-3, seed=2)
+    P1, _ = S.predict(net, Xte)
+    e1 = S.error_stats(P1, Yte)
+    print("loss", hist[0], hist[-1], "untrained", e0, "trained", e1, flush=True)
+    assert hist[-1] < 0.5 * hist[0]
+    assert e1["depth_rmse_m"] < e0["depth_rmse_m"]
+    res = S.screen(net, scs_te, Xte, heldout_frac=0.1, seed=3, chunk=48)
+    run = res["solver_runs"]
+    # where the solver ran, the result is the solver's, bit for bit
+    np.testing.assert_array_equal(res["final"][run], Yte[run])
+    np.testing.assert_array_equal(res["pred"], P1)
+    q = S.asset_depth(P1[:, 0])
+    near = np.abs(q - S.THRESHOLD) < S.MARGIN
+    np.testing.assert_array_equal(res["fallback"], near)
+    assert set(np.flatnonzero(near | res["heldout"])) == set(run)
+    # decisions: the solver's where it ran for a fallback, else the network's
+    truth = S.asset_depth(Yte[:, 0]) > S.THRESHOLD
+    np.testing.assert_array_equal(res["flooded"][near], truth[near])
+    np.testing.assert_array_equal(res["flooded"][~near], (q > S.THRESHOLD)[~near])
+    st2 = res["solver"]
+    screened_s = res["infer_s"] + st2["setup_s"] + st2["run_s"] + st2["copy_s"]
+    gain = plain_s / screened_s
+    print("fallback", int(near.sum()), "held out", int(res["heldout"].sum()), "gain", gain, flush=True)
+    assert gain > 1.0
+    net.close()
