@@ -211,17 +211,52 @@ def generate(seeds, side=SIDE, tf=TF, workers=1):
     return [r[0] for r in res], np.stack([r[1] for r in res])
 
 
+DT_FLOOR = 1e-3         # s; a member whose dt falls below it has blown up
+
+
+def _run_guarded(e, part, log, t_start, check_every=2048, max_steps=400000):
+    """Steps the ensemble until every member has ended.  A member whose dt
+    falls below DT_FLOOR (or whose time stops being finite) is ended where it
+    is and reported as failed: the scheme itself can blow up (seed 10441 of
+    the test set does at t = 1569.5 s in the FP64 oracle exactly as on the
+    device), and one such member would otherwise hold the whole batch at a
+    vanishing dt.  Members still running after max_steps are failed too."""
+    import time
+    steps, failed = 0, set()
+    last = time.perf_counter()
+    while True:
+        e.step(check_every)
+        steps += check_every
+        times = [g.time for g in e.members]
+        for i, ((t, dt), s) in enumerate(zip(times, part)):
+            if i in failed or t >= s["tf"]:
+                continue
+            if not (dt >= DT_FLOOR) or not np.isfinite(t):
+                failed.add(i)
+                e.set_tf(i, t if np.isfinite(t) and t > 0 else 1e-300)
+        if e.finished() == len(part):
+            return steps, sorted(failed)
+        if steps >= max_steps:
+            failed |= {i for i, ((t, _), s) in enumerate(zip(times, part)) if not t >= s["tf"]}
+            return steps, sorted(failed)
+        if log and time.perf_counter() - last > 15.0:
+            last = time.perf_counter()
+            log(f"  ... {steps} steps, {e.finished()}/{len(part)} members ended "
+                f"({last - t_start:.0f} s)")
+
+
 def solve(scs, chunk=256, device=0, log=None):
     """The plain batched solver on the scenarios, `chunk` members per
     ensemble.  Returns labels (n, 2, side, side) float32 -- maximum depth (m)
     and arrival time (h; never wet = tf) -- and timings: setup_s (grid
-    creation, geometry, uploads), run_s (the stepping), copy_s."""
+    creation, geometry, uploads), run_s (the stepping), copy_s.  The labels
+    of members that blew up (stats["failed"]) are NaN."""
     import time
     n = len(scs)
     side = scs[0]["side"]
     hw = side * side
     labels = np.empty((n, 2, side, side), np.float32)
-    stats = {"setup_s": 0.0, "run_s": 0.0, "copy_s": 0.0, "steps": 0}
+    stats = {"setup_s": 0.0, "run_s": 0.0, "copy_s": 0.0, "steps": 0, "failed": []}
     for c0 in range(0, n, chunk):
         part = scs[c0:c0 + chunk]
         t0 = time.perf_counter()
@@ -235,7 +270,8 @@ def solve(scs, chunk=256, device=0, log=None):
             e.set_tf(i, s["tf"])
         e.members[0].sync()
         t1 = time.perf_counter()
-        stats["steps"] = max(stats["steps"], e.run(check_every=64))
+        steps, failed = _run_guarded(e, part, log, t1)
+        stats["steps"] = max(stats["steps"], steps)
         e.members[0].sync()
         t2 = time.perf_counter()
         lab = kp.DeviceArray((len(part), 2, side, side), device)
@@ -249,7 +285,9 @@ def solve(scs, chunk=256, device=0, log=None):
         t3 = time.perf_counter()
         tf = np.array([s["tf"] for s in part], np.float32)[:, None, None]
         arr[:, 1] = np.where(arr[:, 1] < 0.0, tf, arr[:, 1]) / 3600.0
+        arr[failed] = np.nan
         labels[c0:c0 + len(part)] = arr
+        stats["failed"] += [c0 + i for i in failed]
         stats["setup_s"] += t1 - t0
         stats["run_s"] += t2 - t1
         stats["copy_s"] += t3 - t2
@@ -356,14 +394,18 @@ def screen(net, scs, X, heldout_frac=0.05, seed=1, chunk=256, log=None):
     held[rng.choice(len(scs), max(1, int(round(heldout_frac * len(scs)))), replace=False)] = True
     run = np.flatnonzero(near | held)
     truth, st = solve([scs[i] for i in run], chunk=chunk, device=net.desc.device, log=log)
+    ok = ~np.isnan(truth).any(axis=(1, 2, 3))      # solver runs that did not blow up
     final = pred.copy()
-    final[run] = truth
+    final[run[ok]] = truth[ok]
+    solved = np.zeros(len(scs), bool)
+    solved[run[ok]] = True
     pos = {i: k for k, i in enumerate(run)}
-    hk = np.array([pos[i] for i in np.flatnonzero(held)])
-    err = error_stats(pred[held], truth[hk])
-    decided = np.where(near, asset_depth(final[:, 0]) > THRESHOLD, q > THRESHOLD)
+    hk = np.array([pos[i] for i in np.flatnonzero(held & solved)], dtype=int)
+    err = error_stats(pred[held & solved], truth[hk]) if len(hk) else None
+    decided = np.where(near & solved, asset_depth(final[:, 0]) > THRESHOLD, q > THRESHOLD)
     return {"pred": pred, "final": final, "flooded": decided, "fallback": near, "heldout": held,
-            "solver_runs": run, "truth": truth, "infer_s": t_inf, "solver": st, "errors": err}
+            "solver_runs": run, "solved": solved, "truth": truth, "infer_s": t_inf, "solver": st,
+            "errors": err}
 
 
 def prepare(n_train, n_test, train_seed0=5000, test_seed0=9000, workers=16, side=SIDE, tf=TF):
@@ -395,6 +437,8 @@ def run(n_train=4000, n_test=2048, epochs=20, batch=16, lr=1e-3, train_seed0=500
     log(f"generated {n_train} + {n_test} scenarios in {gen_s:.1f} s")
     Ytr, st_tr = solve(scs_tr, chunk=chunk, device=device, log=log)
     del scs_tr
+    keep = ~np.isnan(Ytr).any(axis=(1, 2, 3))      # drop runs that blew up
+    Xtr, Ytr = Xtr[keep], Ytr[keep]
     side = scs_te[0]["side"]
     net = UNet(side, side, len(FEATURES), 2, 32, 4, max_batch=batch, seed=1, device=device)
     net.set_loss_weights([1.0, 0.25])
@@ -413,6 +457,7 @@ def run(n_train=4000, n_test=2048, epochs=20, batch=16, lr=1e-3, train_seed0=500
         "train_s": train_s, "train_loss": hist,
         "train_tflops": train_flops * epochs * (n_train // batch) / max(train_s, 1e-9) / 1e12,
         "infer_s": res["infer_s"], "infer_tflops": fwd_flops / batch * n_test / max(res["infer_s"], 1e-9) / 1e12,
+        "train_solver_failed": len(st_tr["failed"]), "screen_solver_failed": len(st["failed"]),
         "fallback": int(res["fallback"].sum()), "heldout": int(res["heldout"].sum()), "solver_runs": runs,
         "screened_s": screened_s, "screened_per_hour": n_test / screened_s * 3600.0,
         "heldout_errors": res["errors"],
@@ -420,13 +465,15 @@ def run(n_train=4000, n_test=2048, epochs=20, batch=16, lr=1e-3, train_seed0=500
     if solve_all:
         Yte, st_all = solve(scs_te, chunk=chunk, device=device, log=log)
         plain_s = st_all["setup_s"] + st_all["run_s"] + st_all["copy_s"]
-        truth_flood = asset_depth(Yte[:, 0]) > THRESHOLD
+        ok = ~np.isnan(Yte).any(axis=(1, 2, 3))
+        truth_flood = asset_depth(Yte[ok, 0]) > THRESHOLD
         out.update({
             "plain_s": plain_s, "plain_per_hour": n_test / plain_s * 3600.0,
+            "plain_solver_failed": int((~ok).sum()),
             "gain": (n_test / screened_s) / (n_test / plain_s),
-            "all_errors_surrogate": error_stats(res["pred"], Yte),
-            "all_errors_screened": error_stats(res["final"], Yte),
-            "decision_accuracy": float(np.mean(res["flooded"] == truth_flood)),
+            "all_errors_surrogate": error_stats(res["pred"][ok], Yte[ok]),
+            "all_errors_screened": error_stats(res["final"][ok], Yte[ok]),
+            "decision_accuracy": float(np.mean(res["flooded"][ok] == truth_flood)),
             "flooded_frac": float(truth_flood.mean()),
         })
         # the fallback runs alone (held-out runs as a validation cost apart)
