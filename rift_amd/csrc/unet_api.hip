@@ -29,6 +29,12 @@ int fail(int code, const std::string& msg) { return rift::set_last_error(code, m
             return fail(RIFT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+#define GEMM(desc, stream)                                                         \
+    do {                                                                           \
+        if (!launch_gemm((desc), (stream)))                                        \
+            return fail(RIFT_E_UNSUPPORTED, "rift_unet: layer shape not supported"); \
+    } while (0)
+
 enum Kind { kConv3 = 0, kUp2 = 1, kHead = 2 };
 
 struct Layer {
@@ -240,6 +246,7 @@ WGrad conv3_wgrad(const rift_unet* n, const Layer& ly, int B, int l, const bf16*
     w.N = w.cout = ly.cout;
     w.gy = gy;
     w.dw = n->g + ly.w_off;
+    w.db = n->g + ly.b_off;
     return w;
 }
 
@@ -304,6 +311,7 @@ WGrad up_wgrad(const rift_unet* n, const Layer& ly, int B, int l, const bf16* sr
     w.ho = n->H(l);
     w.wo = n->W(l);
     w.dw = n->g + ly.w_off;
+    w.db = n->g + ly.b_off;
     return w;
 }
 
@@ -315,15 +323,15 @@ int forward(rift_unet* n, const float* x, int B, float* y)
     for (int l = 0; l < L; ++l) {
         const bf16* in = l == 0 ? n->x32 : n->pool[l - 1];
         const int cin = l == 0 ? 32 : n->ch[l - 1];
-        launch_gemm(conv3_fwd(n, n->enc(l, 0), B, l, in, cin, nullptr, 0, n->enc_a[l]), s);
-        launch_gemm(conv3_fwd(n, n->enc(l, 1), B, l, n->enc_a[l], n->ch[l], nullptr, 0, n->enc_b[l]), s);
+        GEMM(conv3_fwd(n, n->enc(l, 0), B, l, in, cin, nullptr, 0, n->enc_a[l]), s);
+        GEMM(conv3_fwd(n, n->enc(l, 1), B, l, n->enc_a[l], n->ch[l], nullptr, 0, n->enc_b[l]), s);
         if (l < L - 1) launch_pool2(n->enc_b[l], n->pool[l], B, n->H(l), n->W(l), n->ch[l], s);
     }
     const bf16* a = n->enc_b[L - 1];
     for (int l = L - 2; l >= 0; --l) {
-        launch_gemm(up_fwd(n, n->upl(l), B, l, a, n->up[l]), s);
-        launch_gemm(conv3_fwd(n, n->dec(l, 0), B, l, n->up[l], n->ch[l], n->enc_b[l], n->ch[l], n->dec_a[l]), s);
-        launch_gemm(conv3_fwd(n, n->dec(l, 1), B, l, n->dec_a[l], n->ch[l], nullptr, 0, n->dec_b[l]), s);
+        GEMM(up_fwd(n, n->upl(l), B, l, a, n->up[l]), s);
+        GEMM(conv3_fwd(n, n->dec(l, 0), B, l, n->up[l], n->ch[l], n->enc_b[l], n->ch[l], n->dec_a[l]), s);
+        GEMM(conv3_fwd(n, n->dec(l, 1), B, l, n->dec_a[l], n->ch[l], nullptr, 0, n->dec_b[l]), s);
         a = n->dec_b[l];
     }
     const Layer& hd = n->head();
@@ -339,7 +347,7 @@ int forward(rift_unet* n, const float* x, int B, float* y)
     g.bias = n->p + hd.b_off;
     g.dstf = y;
     g.nvalid = hd.cout;
-    launch_gemm(g, s);
+    GEMM(g, s);
     UCHK(hipGetLastError());
     return RIFT_OK;
 }
@@ -350,7 +358,6 @@ int backward(rift_unet* n, const float* y, const float* tgt, int B)
     hipStream_t s = n->stream;
     const int L = n->L;
     const int C = n->desc.out_channels;
-    auto P = [&](int l) { return (long long)B * n->H(l) * n->W(l); };
     launch_mse(y, tgt, n->lossw, B, C, n->H(0), n->W(0), 32, n->ghead, n->loss, s);
     {   // head
         const Layer& hd = n->head();
@@ -364,8 +371,8 @@ int backward(rift_unet* n, const float* y, const float* tgt, int B)
         w.N = w.cout = 32;
         w.gy = n->ghead;
         w.dw = n->g + hd.w_off;
+        w.db = n->g + hd.b_off;
         launch_wgrad(w, s);
-        launch_colsum(n->ghead, P(0), 32, C, n->g + hd.b_off, s);
         Gemm g;
         g.src0 = n->ghead;
         g.c0 = 32;
@@ -377,25 +384,22 @@ int backward(rift_unet* n, const float* y, const float* tgt, int B)
         g.N = g.cout = n->ch[0];
         g.mask = n->dec_b[0];
         g.dst0 = n->gdec_b[0];
-        launch_gemm(g, s);
+        GEMM(g, s);
     }
     for (int l = 0; l <= L - 2; ++l) {
         const int c = n->ch[l];
         const Layer& lb = n->dec(l, 1);
         launch_wgrad(conv3_wgrad(n, lb, B, l, n->dec_a[l], c, nullptr, 0, n->gdec_b[l]), s);
-        launch_colsum(n->gdec_b[l], P(l), c, c, n->g + lb.b_off, s);
-        launch_gemm(conv3_dgrad(n, lb, B, l, n->gdec_b[l], n->dec_a[l], n->gdec_a[l], nullptr, 0), s);
+        GEMM(conv3_dgrad(n, lb, B, l, n->gdec_b[l], n->dec_a[l], n->gdec_a[l], nullptr, 0), s);
         const Layer& la = n->dec(l, 0);
         launch_wgrad(conv3_wgrad(n, la, B, l, n->up[l], c, n->enc_b[l], c, n->gdec_a[l]), s);
-        launch_colsum(n->gdec_a[l], P(l), c, c, n->g + la.b_off, s);
-        launch_gemm(conv3_dgrad(n, la, B, l, n->gdec_a[l], nullptr, n->gup[l], n->gskip[l], c), s);
+        GEMM(conv3_dgrad(n, la, B, l, n->gdec_a[l], nullptr, n->gup[l], n->gskip[l], c), s);
         const Layer& lu = n->upl(l);
         const bool top = l + 1 == L - 1;
         const bf16* src = top ? n->enc_b[L - 1] : n->dec_b[l + 1];
         bf16* gsrc = top ? n->genc_b[L - 1] : n->gdec_b[l + 1];
         launch_wgrad(up_wgrad(n, lu, B, l, src, n->gup[l]), s);
-        launch_colsum(n->gup[l], P(l), c, c, n->g + lu.b_off, s);
-        launch_gemm(up_dgrad(n, lu, B, l, n->gup[l], src, gsrc), s);
+        GEMM(up_dgrad(n, lu, B, l, n->gup[l], src, gsrc), s);
     }
     for (int l = L - 1; l >= 0; --l) {
         const int c = n->ch[l];
@@ -403,13 +407,11 @@ int backward(rift_unet* n, const float* y, const float* tgt, int B)
             launch_pool2_back(n->gpool[l], n->gskip[l], n->enc_b[l], n->genc_b[l], B, n->H(l), n->W(l), c, s);
         const Layer& lb = n->enc(l, 1);
         launch_wgrad(conv3_wgrad(n, lb, B, l, n->enc_a[l], c, nullptr, 0, n->genc_b[l]), s);
-        launch_colsum(n->genc_b[l], P(l), c, c, n->g + lb.b_off, s);
-        launch_gemm(conv3_dgrad(n, lb, B, l, n->genc_b[l], n->enc_a[l], n->genc_a[l], nullptr, 0), s);
+        GEMM(conv3_dgrad(n, lb, B, l, n->genc_b[l], n->enc_a[l], n->genc_a[l], nullptr, 0), s);
         const Layer& la = n->enc(l, 0);
         const bf16* in = l == 0 ? n->x32 : n->pool[l - 1];
         launch_wgrad(conv3_wgrad(n, la, B, l, in, la.dcin, nullptr, 0, n->genc_a[l]), s);
-        launch_colsum(n->genc_a[l], P(l), c, c, n->g + la.b_off, s);
-        if (l > 0) launch_gemm(conv3_dgrad(n, la, B, l, n->genc_a[l], nullptr, n->gpool[l - 1], nullptr, 0), s);
+        if (l > 0) GEMM(conv3_dgrad(n, la, B, l, n->genc_a[l], nullptr, n->gpool[l - 1], nullptr, 0), s);
     }
     UCHK(hipGetLastError());
     return RIFT_OK;

@@ -51,10 +51,40 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // Lane (r = lane % 32, kh = lane / 32): A fragment = row r, K values
 // [8 kh, 8 kh + 8) of the 16-wide step; B fragment = column r, same K; the
 // 16 accumulators of tile (rm, rn) hold rows 8 q + 4 kh + e (acc[4 q + e]) of
-// column r.
-template <int WM, int WN, int RM, int RN>
+// column r.  The K loop is one flat sequence of 16-wide steps (tap-major,
+// channel-minor across both sources) with the fragments of step q+1 loaded
+// while the MFMAs of step q run.
+
+// Tap patterns: 3x3 (offsets -1..1), 3x3 mirrored (input gradients), one
+// tap, and 2x2 at stride 2 (input gradient of the transposed convolution).
+enum TapKind { kTap3 = 0, kTap3Mirror = 1, kTap1 = 2, kTap2x2 = 3 };
+
+template <int TK>
+struct TapGeo {
+    static constexpr int NT = TK == kTap1 ? 1 : (TK == kTap2x2 ? 4 : 9);
+    static constexpr int S = TK == kTap2x2 ? 2 : 1;
+    __device__ static __forceinline__ void off(int t, int& ty, int& tx)
+    {
+        if constexpr (TK == kTap3) {
+            ty = t / 3 - 1;
+            tx = t - 3 * (t / 3) - 1;
+        } else if constexpr (TK == kTap3Mirror) {
+            ty = 1 - t / 3;
+            tx = 1 - (t - 3 * (t / 3));
+        } else if constexpr (TK == kTap2x2) {
+            ty = t >> 1;
+            tx = t & 1;
+        } else {
+            ty = 0;
+            tx = 0;
+        }
+    }
+};
+
+template <int TK, int WM, int WN, int RM, int RN>
 __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 {
+    using G = TapGeo<TK>;
     constexpr int BM = 32 * WM * RM, BN = 32 * WN * RN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave % WM, wn = wave / WM;
@@ -73,8 +103,8 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
         const int mm = pv[a] ? m : 0;
         pb[a] = mm / hw;
         const int rem = mm - pb[a] * hw;
-        pi[a] = rem / g.wg;
-        pj[a] = rem - pi[a] * g.wg;
+        pi[a] = G::S * (rem / g.wg);
+        pj[a] = G::S * (rem - (rem / g.wg) * g.wg);
     }
     f32x16 acc[RM][RN];
 #pragma unroll
@@ -89,36 +119,46 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 #pragma unroll
     for (int b = 0; b < RN; ++b) wrow[b] = g.w + (size_t)(n0 + b * 32 + r) * g.K + kh * 8;
 
-    for (int t = 0; t < g.ntaps; ++t) {
-        long long off[RM];
-        bool ok[RM];
+    auto load = [&](int t, int kc, bf16x8(&fa)[RM], bf16x8(&fb)[RN]) {
+        int ty, tx;
+        G::off(t, ty, tx);
+        const bool second = kc >= g.c0;
+        const bf16* src = second ? g.src1 : g.src0;
+        const int cs = second ? g.c1 : g.c0;
+        const int c = (second ? kc - g.c0 : kc) + kh * 8;
 #pragma unroll
         for (int a = 0; a < RM; ++a) {
-            const int si = g.stride * pi[a] + g.ty[t], sj = g.stride * pj[a] + g.tx[t];
-            ok[a] = pv[a] && si >= 0 && si < g.hs && sj >= 0 && sj < g.ws;
-            off[a] = ((long long)pb[a] * g.hs + si) * g.ws + sj;
+            const int si = pi[a] + ty, sj = pj[a] + tx;
+            const bool ok = pv[a] && (unsigned)si < (unsigned)g.hs && (unsigned)sj < (unsigned)g.ws;
+            const size_t pix = (size_t)((pb[a] * g.hs + si) * g.ws + sj);
+            fa[a] = ok ? load8(src + pix * cs + c) : zero8();
         }
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const bf16* src = s ? g.src1 : g.src0;
-            const int cs = s ? g.c1 : g.c0;
-            const int kb = t * cin + (s ? g.c0 : 0);
-            const bf16* ap[RM];
-#pragma unroll
-            for (int a = 0; a < RM; ++a) ap[a] = src + off[a] * cs + kh * 8;
-            for (int c = 0; c < cs; c += 16) {
-                bf16x8 fa[RM], fb[RN];
-#pragma unroll
-                for (int a = 0; a < RM; ++a) fa[a] = ok[a] ? load8(ap[a] + c) : zero8();
-#pragma unroll
-                for (int b = 0; b < RN; ++b) fb[b] = load8(wrow[b] + kb + c);
-#pragma unroll
-                for (int a = 0; a < RM; ++a)
-#pragma unroll
-                    for (int b = 0; b < RN; ++b)
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
-            }
+        for (int b = 0; b < RN; ++b) fb[b] = load8(wrow[b] + t * cin + kc);
+    };
+
+    const int Q = G::NT * (cin >> 4);
+    bf16x8 ca[RM], cb[RN], na[RM], nb[RN];
+    load(0, 0, ca, cb);
+    int t = 0, kc = 0;
+    for (int q = 0; q < Q; ++q) {
+        int t1 = t, kc1 = kc + 16;
+        if (kc1 == cin) {
+            kc1 = 0;
+            ++t1;
         }
+        if (q + 1 < Q) load(t1, kc1, na, nb);
+#pragma unroll
+        for (int a = 0; a < RM; ++a)
+#pragma unroll
+            for (int b = 0; b < RN; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ca[a], cb[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < RM; ++a) ca[a] = na[a];
+#pragma unroll
+        for (int b = 0; b < RN; ++b) cb[b] = nb[b];
+        t = t1;
+        kc = kc1;
     }
 
     // epilogue: bias, ReLU, mask, store
@@ -129,6 +169,13 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
         const int sub = n / g.cout, co = n - sub * g.cout;
         const int su = sub / g.ups, sv = sub - su * g.ups;
         const float bias = g.bias ? g.bias[co] : 0.f;
+        bf16* dst = g.dst0;
+        int dstride = g.dst1 ? g.dsplit : g.cout, dco = co;
+        if (g.dst1 && co >= g.dsplit) {
+            dst = g.dst1;
+            dstride = g.cout - g.dsplit;
+            dco = co - g.dsplit;
+        }
 #pragma unroll
         for (int a = 0; a < RM; ++a)
 #pragma unroll
@@ -137,19 +184,23 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
                 for (int e = 0; e < 4; ++e) {
                     const int m = m0 + a * 32 + 8 * q + 4 * kh + e;
                     if (m >= M) continue;
-                    const int ib = m / hw, rem = m - ib * hw;
-                    const int i = rem / g.wg, j = rem - i * g.wg;
-                    const int oi = g.ups * i + su, oj = g.ups * j + sv;
-                    const long long op = ((long long)ib * g.ho + oi) * g.wo + oj;
+                    long long op = m;           // output pixel (same grid when ups == 1)
+                    int ib = 0, rem = m;
+                    if (g.ups != 1 || g.dstf) {
+                        ib = m / hw;
+                        rem = m - ib * hw;
+                    }
+                    if (g.ups != 1) {
+                        const int i = rem / g.wg, j = rem - i * g.wg;
+                        op = ((long long)ib * g.ho + (g.ups * i + su)) * g.wo + (g.ups * j + sv);
+                    }
                     float v = acc[a][b][4 * q + e] + bias;
                     if (g.relu) v = fmaxf(v, 0.f);
                     if (g.mask && !(bf2f(g.mask[op * g.cout + co]) > 0.f)) v = 0.f;
                     if (g.dstf) {
-                        if (co < g.nvalid) g.dstf[(((long long)ib * g.nvalid + co) * g.ho + oi) * g.wo + oj] = v;
-                    } else if (g.dst1 && co >= g.dsplit) {
-                        g.dst1[op * (g.cout - g.dsplit) + (co - g.dsplit)] = f2bf(v);
+                        if (co < g.nvalid) g.dstf[((long long)ib * g.nvalid + co) * hw + rem] = v;
                     } else {
-                        g.dst0[op * (g.dst1 ? g.dsplit : g.cout) + co] = f2bf(v);
+                        dst[op * dstride + dco] = f2bf(v);
                     }
                 }
     }
@@ -161,7 +212,9 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 // same output mapping.  Block tile 32 (n) x 128 (k), one 32 x 32 tile per
 // wave; pixels in chunks of 64, both operands staged transposed in LDS
 // (rows of 64 pixels, padded to 72).  Partial sums go to dW with FP32
-// atomics, so the summation order varies between runs.
+// atomics, so the summation order varies between runs.  The blocks of the
+// first K tile also sum the staged output gradient over pixels: the bias
+// gradient (for a transposed conv, every sub-pixel adds to its channel).
 constexpr int kWgPix = 64, kWgPad = 72;
 
 __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
@@ -191,6 +244,8 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
         kc[q] = ksrc[q] ? c - w.c0 : c;
     }
     const bool mine = k0 + wave * 32 < w.K;
+    const bool bias = w.db != nullptr && blockIdx.x == 0;
+    float bsum = 0.f;   // channel tid / 8, pixels (tid % 8) * 8 .. + 8 of each chunk
     f32x16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.f;
@@ -232,6 +287,12 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
                 sA[(q * 32 + cv * 8 + e) * kWgPad + px] = (bf16)(wv[e >> 1] >> ((e & 1) * 16));
         }
         __syncthreads();
+        if (bias) {
+            const uint4 u = *reinterpret_cast<const uint4*>(&sG[(tid >> 3) * kWgPad + (tid & 7) * 8]);
+            const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) bsum += bf2f((bf16)(wv[e] & 0xffffu)) + bf2f((bf16)(wv[e] >> 16));
+        }
         if (mine) {
 #pragma unroll
             for (int st = 0; st < kWgPix / 16; ++st) {
@@ -244,6 +305,12 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
         }
         __syncthreads();
     }
+    if (bias) {
+        bsum += __shfl_xor(bsum, 1);
+        bsum += __shfl_xor(bsum, 2);
+        bsum += __shfl_xor(bsum, 4);
+        if ((tid & 7) == 0 && bsum != 0.f) unsafeAtomicAdd(&w.db[co0 + (tid >> 3)], bsum);
+    }
     if (!mine) return;
     const int k = k0 + wave * 32 + r;
 #pragma unroll
@@ -254,20 +321,6 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
             const float v = acc[4 * q + e];
             if (v != 0.f) unsafeAtomicAdd(&w.dw[(size_t)n * w.K + k], v);
         }
-}
-
-// Column sums of a [P][C] tensor into db[C] (bias gradients); C <= 256.
-__global__ __launch_bounds__(256) void k_colsum(const bf16* __restrict__ x, long long P, int C, int nvalid,
-                                               int chunk, float* __restrict__ db)
-{
-    const int per = 256 / C;
-    const int c = threadIdx.x % C, lane_p = threadIdx.x / C;
-    if (lane_p >= per) return;
-    const long long p0 = (long long)blockIdx.x * chunk;
-    const long long p1 = min(P, p0 + chunk);
-    float s = 0.f;
-    for (long long p = p0 + lane_p; p < p1; p += per) s += bf2f(x[p * C + c]);
-    if (c < nvalid && s != 0.f) unsafeAtomicAdd(&db[c], s);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,20 +494,34 @@ __global__ void k_wt_transpose(const bf16* __restrict__ in, bf16* __restrict__ o
 
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
-template <int WM, int WN, int RM, int RN>
+template <int TK, int WM, int WN, int RM, int RN>
 static void launch_gemm_t(const Gemm& g, hipStream_t s)
 {
     constexpr int BM = 32 * WM * RM, BN = 32 * WN * RN;
     const long long M = (long long)g.batch * g.hg * g.wg;
     dim3 grid(cdiv(M, BM), cdiv(g.N, BN));
-    hipLaunchKernelGGL((k_gemm<WM, WN, RM, RN>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((k_gemm<TK, WM, WN, RM, RN>), grid, dim3(256), 0, s, g);
 }
 
-void launch_gemm(const Gemm& g, hipStream_t s)
+template <int TK>
+static void launch_gemm_tk(const Gemm& g, hipStream_t s)
 {
-    if (g.N % 128 == 0) launch_gemm_t<2, 2, 2, 2>(g, s);
-    else if (g.N % 64 == 0) launch_gemm_t<2, 2, 2, 1>(g, s);
-    else launch_gemm_t<4, 1, 2, 1>(g, s);
+    if (g.N % 128 == 0) launch_gemm_t<TK, 2, 2, 2, 2>(g, s);
+    else if (g.N % 64 == 0) launch_gemm_t<TK, 4, 1, 2, 2>(g, s);
+    else launch_gemm_t<TK, 4, 1, 4, 1>(g, s);
+}
+
+// The tap pattern is given by ntaps, stride and the first offset; the
+// kernels derive the offsets at compile time.
+bool launch_gemm(const Gemm& g, hipStream_t s)
+{
+    if (g.N % 32 || (g.c0 | g.c1) % 32) return false;
+    if (g.ntaps == 9 && g.stride == 1 && g.ty[0] == -1 && g.tx[0] == -1) launch_gemm_tk<kTap3>(g, s);
+    else if (g.ntaps == 9 && g.stride == 1 && g.ty[0] == 1 && g.tx[0] == 1) launch_gemm_tk<kTap3Mirror>(g, s);
+    else if (g.ntaps == 1 && g.stride == 1 && g.ty[0] == 0 && g.tx[0] == 0) launch_gemm_tk<kTap1>(g, s);
+    else if (g.ntaps == 4 && g.stride == 2 && g.ty[0] == 0 && g.tx[0] == 0) launch_gemm_tk<kTap2x2>(g, s);
+    else return false;
+    return true;
 }
 
 void launch_wgrad(WGrad w, hipStream_t s)
@@ -466,12 +533,6 @@ void launch_wgrad(WGrad w, hipStream_t s)
     w.chunk = cdiv(cdiv(M, splits), kWgPix) * kWgPix;
     dim3 grid(kt, nt, cdiv(M, w.chunk));
     hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, s, w);
-}
-
-void launch_colsum(const bf16* x, long long P, int C, int nvalid, float* db, hipStream_t s)
-{
-    const int chunk = 4096;
-    hipLaunchKernelGGL(k_colsum, dim3(cdiv(P, chunk)), dim3(256), 0, s, x, P, C, nvalid, chunk, db);
 }
 
 void launch_pool2(const bf16* x, bf16* y, int B, int H, int W, int C, hipStream_t s)

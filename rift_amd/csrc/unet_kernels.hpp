@@ -48,12 +48,13 @@ struct WGrad {
     const bf16* gy = nullptr;
     int cout = 0, ups = 1, ho = 0, wo = 0;
     float* dw = nullptr;
+    float* db = nullptr;         // bias gradient [cout] accumulated too, or null
     int chunk = 0;               // pixels per block (set by the launcher)
 };
 
-void launch_gemm(const Gemm& g, hipStream_t s);
+// False (nothing launched) for a tap pattern or shape the kernels lack.
+bool launch_gemm(const Gemm& g, hipStream_t s);
 void launch_wgrad(WGrad w, hipStream_t s);
-void launch_colsum(const bf16* x, long long P, int C, int nvalid, float* db, hipStream_t s);
 void launch_pool2(const bf16* x, bf16* y, int B, int H, int W, int C, hipStream_t s);
 void launch_pool2_back(const bf16* dP, const bf16* dskip, const bf16* E, bf16* dE, int B, int H, int W, int C,
                        hipStream_t s);
