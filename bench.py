@@ -163,6 +163,11 @@ def surrogate_leg(data, epochs):
             "all_errors_screened", "decision_accuracy", "flooded_frac"]
     out = {k: r.get(k) for k in keep}
     out["final_train_loss"] = r["train_loss"][-1] if r["train_loss"] else None
+    st = surrogate.step_timing(16)
+    out["step_timing"] = st
+    out["roofline"] = {"bound": "mfma", "kernel": "training step (k_gemm, k_wgrad, ...)",
+                       "achieved": st["train_tflops"], "peak": surrogate.BF16_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": st["train_tflops"] / surrogate.BF16_PEAK_TFLOPS}
     out["workload"] = (f"U-Net 4 levels x 32 base ch, BF16 MFMA; {r['n_train']} training runs / "
                        f"{r['n_test']} test scenarios at 256x256 (reduced from 4000 / 2048), "
                        "threshold 0.3 m +- 0.1 m at the central 8x8 cells, 5% held out")

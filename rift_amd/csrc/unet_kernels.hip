@@ -51,9 +51,7 @@ __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, ma
 // Lane (r = lane % 32, kh = lane / 32): A fragment = row r, K values
 // [8 kh, 8 kh + 8) of the 16-wide step; B fragment = column r, same K; the
 // 16 accumulators of tile (rm, rn) hold rows 8 q + 4 kh + e (acc[4 q + e]) of
-// column r.  The K loop is one flat sequence of 16-wide steps (tap-major,
-// channel-minor across both sources) with the fragments of step q+1 loaded
-// while the MFMAs of step q run.
+// column r.
 
 // Tap patterns: 3x3 (offsets -1..1), 3x3 mirrored (input gradients), one
 // tap, and 2x2 at stride 2 (input gradient of the transposed convolution).
@@ -81,24 +79,39 @@ struct TapGeo {
     }
 };
 
+// K advances in chunks of 32 = one tap of one 32-channel block of one
+// source.  A chunk's A tile (BM pixels x 32) and B tile (BN columns x 32) are
+// loaded from global memory into registers one chunk ahead, stored to LDS
+// (double-buffered, rows padded to 40 elements so that the 16-byte fragment
+// reads of 8 consecutive rows hit disjoint banks), and the waves read their
+// MFMA fragments from LDS.
+constexpr int kGemmBK = 32, kGemmLd = 40;
+
 template <int TK, int WM, int WN, int RM, int RN>
 __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 {
     using G = TapGeo<TK>;
     constexpr int BM = 32 * WM * RM, BN = 32 * WN * RN;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int PA = BM / 64;                 // A rows per thread and chunk
+    constexpr int PB = (BN + 63) / 64;          // B rows per thread and chunk
+    __shared__ __attribute__((aligned(16))) bf16 sA[2][BM * kGemmLd];
+    __shared__ __attribute__((aligned(16))) bf16 sB[2][BN * kGemmLd];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
     const int r = lane & 31, kh = lane >> 5;
     const int hw = g.hg * g.wg;
     const int M = g.batch * hw;
-    const int m0 = blockIdx.x * BM + wm * 32 * RM;
-    const int n0 = blockIdx.y * BN + wn * 32 * RN;
+    const int bm0 = blockIdx.x * BM, bn0 = blockIdx.y * BN;
+    const int m0 = bm0 + wm * 32 * RM;
+    const int n0 = bn0 + wn * 32 * RN;
+    const int lrow = tid >> 2, lq = (tid & 3) * 8;   // loader: row in a 64-row pass, channel octet
 
-    int pb[RM], pi[RM], pj[RM];
-    bool pv[RM];
+    // loader rows of the A tile: pixel of each
+    int pb[PA], pi[PA], pj[PA];
+    bool pv[PA];
 #pragma unroll
-    for (int a = 0; a < RM; ++a) {
-        const int m = m0 + a * 32 + r;
+    for (int a = 0; a < PA; ++a) {
+        const int m = bm0 + a * 64 + lrow;
         pv[a] = m < M;
         const int mm = pv[a] ? m : 0;
         pb[a] = mm / hw;
@@ -106,6 +119,42 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
         pi[a] = G::S * (rem / g.wg);
         pj[a] = G::S * (rem - (rem / g.wg) * g.wg);
     }
+    const int cin = g.c0 + g.c1;
+    const int cpt = cin >> 5;                        // chunks per tap
+    const int Q = G::NT * cpt;
+
+    uint4 ra[PA], rb[PB];
+    auto gload = [&](int q) {
+        const int t = q / cpt, k = (q - t * cpt) * kGemmBK;  // tap, channel offset in [0, cin)
+        int ty, tx;
+        G::off(t, ty, tx);
+        const bool second = k >= g.c0;
+        const bf16* src = second ? g.src1 : g.src0;
+        const int cs = second ? g.c1 : g.c0;
+        const int c = (second ? k - g.c0 : k) + lq;
+#pragma unroll
+        for (int a = 0; a < PA; ++a) {
+            const int si = pi[a] + ty, sj = pj[a] + tx;
+            const bool ok = pv[a] && (unsigned)si < (unsigned)g.hs && (unsigned)sj < (unsigned)g.ws;
+            const size_t pix = (size_t)((pb[a] * g.hs + si) * g.ws + sj);
+            ra[a] = ok ? *reinterpret_cast<const uint4*>(src + pix * cs + c) : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int b = 0; b < PB; ++b) {
+            const int n = b * 64 + lrow;
+            if (n < BN) rb[b] = *reinterpret_cast<const uint4*>(g.w + (size_t)(bn0 + n) * g.K + t * cin + k + lq);
+        }
+    };
+    auto swrite = [&](int buf) {
+#pragma unroll
+        for (int a = 0; a < PA; ++a) *reinterpret_cast<uint4*>(&sA[buf][(a * 64 + lrow) * kGemmLd + lq]) = ra[a];
+#pragma unroll
+        for (int b = 0; b < PB; ++b) {
+            const int n = b * 64 + lrow;
+            if (n < BN) *reinterpret_cast<uint4*>(&sB[buf][n * kGemmLd + lq]) = rb[b];
+        }
+    };
+
     f32x16 acc[RM][RN];
 #pragma unroll
     for (int a = 0; a < RM; ++a)
@@ -114,51 +163,33 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 #pragma unroll
             for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.f;
 
-    const int cin = g.c0 + g.c1;
-    const bf16* wrow[RN];
-#pragma unroll
-    for (int b = 0; b < RN; ++b) wrow[b] = g.w + (size_t)(n0 + b * 32 + r) * g.K + kh * 8;
-
-    auto load = [&](int t, int kc, bf16x8(&fa)[RM], bf16x8(&fb)[RN]) {
-        int ty, tx;
-        G::off(t, ty, tx);
-        const bool second = kc >= g.c0;
-        const bf16* src = second ? g.src1 : g.src0;
-        const int cs = second ? g.c1 : g.c0;
-        const int c = (second ? kc - g.c0 : kc) + kh * 8;
-#pragma unroll
-        for (int a = 0; a < RM; ++a) {
-            const int si = pi[a] + ty, sj = pj[a] + tx;
-            const bool ok = pv[a] && (unsigned)si < (unsigned)g.hs && (unsigned)sj < (unsigned)g.ws;
-            const size_t pix = (size_t)((pb[a] * g.hs + si) * g.ws + sj);
-            fa[a] = ok ? load8(src + pix * cs + c) : zero8();
-        }
-#pragma unroll
-        for (int b = 0; b < RN; ++b) fb[b] = load8(wrow[b] + t * cin + kc);
-    };
-
-    const int Q = G::NT * (cin >> 4);
-    bf16x8 ca[RM], cb[RN], na[RM], nb[RN];
-    load(0, 0, ca, cb);
-    int t = 0, kc = 0;
+    gload(0);
+    swrite(0);
+    __syncthreads();
     for (int q = 0; q < Q; ++q) {
-        int t1 = t, kc1 = kc + 16;
-        if (kc1 == cin) {
-            kc1 = 0;
-            ++t1;
-        }
-        if (q + 1 < Q) load(t1, kc1, na, nb);
+        const int buf = q & 1;
+        if (q + 1 < Q) gload(q + 1);
 #pragma unroll
-        for (int a = 0; a < RM; ++a)
+        for (int kk = 0; kk < kGemmBK / 16; ++kk) {
+            bf16x8 fa[RM], fb[RN];
+#pragma unroll
+            for (int a = 0; a < RM; ++a)
+                fa[a] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(
+                                &sA[buf][(wm * 32 * RM + a * 32 + r) * kGemmLd + kk * 16 + kh * 8]));
 #pragma unroll
             for (int b = 0; b < RN; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ca[a], cb[b], acc[a][b], 0, 0, 0);
+                fb[b] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(
+                                &sB[buf][(wn * 32 * RN + b * 32 + r) * kGemmLd + kk * 16 + kh * 8]));
 #pragma unroll
-        for (int a = 0; a < RM; ++a) ca[a] = na[a];
+            for (int a = 0; a < RM; ++a)
 #pragma unroll
-        for (int b = 0; b < RN; ++b) cb[b] = nb[b];
-        t = t1;
-        kc = kc1;
+                for (int b = 0; b < RN; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+        if (q + 1 < Q) swrite(buf ^ 1);
+        __syncthreads();
     }
 
     // epilogue: bias, ReLU, mask, store
@@ -508,7 +539,7 @@ static void launch_gemm_tk(const Gemm& g, hipStream_t s)
 {
     if (g.N % 128 == 0) launch_gemm_t<TK, 2, 2, 2, 2>(g, s);
     else if (g.N % 64 == 0) launch_gemm_t<TK, 4, 1, 2, 2>(g, s);
-    else launch_gemm_t<TK, 4, 1, 4, 1>(g, s);
+    else launch_gemm_t<TK, 4, 1, 2, 1>(g, s);
 }
 
 // The tap pattern is given by ntaps, stride and the first offset; the

@@ -484,6 +484,37 @@ def run(n_train=4000, n_test=2048, epochs=20, batch=16, lr=1e-3, train_seed0=500
     return out
 
 
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 matrix peak
+
+
+def step_timing(batch=16, steps=10, side=SIDE, device=0):
+    """Training-step and forward-pass time of the network at `batch` on
+    random inputs (device resident): ms per step and TFLOP/s."""
+    import time
+    net = UNet(side, side, len(FEATURES), 2, 32, 4, max_batch=batch, seed=1, device=device)
+    rng = np.random.default_rng(0)
+    x = kp.DeviceArray((batch, len(FEATURES), side, side), device)
+    x.upload(rng.standard_normal(x.shape).astype(np.float32))
+    t = kp.DeviceArray((batch, 2, side, side), device)
+    t.upload(rng.random(t.shape).astype(np.float32))
+    y = kp.DeviceArray((batch, 2, side, side), device)
+    for _ in range(3):
+        net.train_step(x, t, batch, 1e-4)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.train_step(x, t, batch, 1e-4)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        net.forward(x, batch, y)
+    t2 = time.perf_counter()
+    f, tr = net.flops(batch)
+    net.close()
+    ms_tr, ms_f = 1e3 * (t1 - t0) / steps, 1e3 * (t2 - t1) / steps
+    return {"batch": batch, "side": side, "train_ms_per_step": ms_tr, "train_tflops": tr / ms_tr / 1e9,
+            "forward_ms": ms_f, "forward_tflops": f / ms_f / 1e9,
+            "forward_gflop_per_scenario": f / batch / 1e9, "train_gflop_per_scenario": tr / batch / 1e9}
+
+
 if __name__ == "__main__":
     import argparse
     import json
@@ -496,7 +527,12 @@ if __name__ == "__main__":
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--step-timing", action="store_true",
+                    help="only time training steps and forward passes at --batch on random inputs")
     a = ap.parse_args()
+    if a.step_timing:
+        print(json.dumps(step_timing(a.batch)))
+        sys.exit(0)
     r = run(a.train, a.test, a.epochs, a.batch, a.lr, workers=a.workers,
             log=lambda m: print("[surrogate] " + m, file=sys.stderr, flush=True))
     txt = json.dumps(r)
