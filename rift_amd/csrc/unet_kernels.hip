@@ -79,23 +79,24 @@ struct TapGeo {
     }
 };
 
-// K advances in chunks of 32 = one tap of one 32-channel block of one
-// source.  A chunk's A tile (BM pixels x 32) and B tile (BN columns x 32) are
-// loaded from global memory into registers one chunk ahead, stored to LDS
-// (double-buffered, rows padded to 40 elements so that the 16-byte fragment
-// reads of 8 consecutive rows hit disjoint banks), and the waves read their
-// MFMA fragments from LDS.
-constexpr int kGemmBK = 32, kGemmLd = 40;
+// K advances in groups of 32 = one tap of one 32-channel block of one
+// source; a stage holds KC groups.  A stage's A tile (BM pixels x 32 KC) and
+// B tile (BN columns x 32 KC) are loaded from global memory into registers
+// one stage ahead, stored to LDS (double-buffered, rows padded by 8 elements
+// so that the 16-byte fragment reads of 8 consecutive rows hit disjoint
+// banks), and the waves read their MFMA fragments from LDS.
+constexpr int kGroup = 32;
 
-template <int TK, int WM, int WN, int RM, int RN>
+template <int TK, int WM, int WN, int RM, int RN, int KC>
 __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 {
     using G = TapGeo<TK>;
     constexpr int BM = 32 * WM * RM, BN = 32 * WN * RN;
-    constexpr int PA = BM / 64;                 // A rows per thread and chunk
-    constexpr int PB = (BN + 63) / 64;          // B rows per thread and chunk
-    __shared__ __attribute__((aligned(16))) bf16 sA[2][BM * kGemmLd];
-    __shared__ __attribute__((aligned(16))) bf16 sB[2][BN * kGemmLd];
+    constexpr int LD = KC * kGroup + 8;         // LDS row stride (elements)
+    constexpr int PA = BM / 64;                 // A rows per thread and group
+    constexpr int PB = (BN + 63) / 64;          // B rows per thread and group
+    __shared__ __attribute__((aligned(16))) bf16 sA[2][BM * LD];
+    __shared__ __attribute__((aligned(16))) bf16 sB[2][BN * LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
     const int r = lane & 31, kh = lane >> 5;
@@ -120,38 +121,51 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
         pj[a] = G::S * (rem - (rem / g.wg) * g.wg);
     }
     const int cin = g.c0 + g.c1;
-    const int cpt = cin >> 5;                        // chunks per tap
-    const int Q = G::NT * cpt;
+    const int cpt = cin >> 5;                        // groups per tap
+    const int QG = G::NT * cpt;                      // groups
+    const int Q = (QG + KC - 1) / KC;                // stages
 
-    uint4 ra[PA], rb[PB];
+    uint4 ra[KC][PA], rb[KC][PB];
     auto gload = [&](int q) {
-        const int t = q / cpt, k = (q - t * cpt) * kGemmBK;  // tap, channel offset in [0, cin)
-        int ty, tx;
-        G::off(t, ty, tx);
-        const bool second = k >= g.c0;
-        const bf16* src = second ? g.src1 : g.src0;
-        const int cs = second ? g.c1 : g.c0;
-        const int c = (second ? k - g.c0 : k) + lq;
 #pragma unroll
-        for (int a = 0; a < PA; ++a) {
-            const int si = pi[a] + ty, sj = pj[a] + tx;
-            const bool ok = pv[a] && (unsigned)si < (unsigned)g.hs && (unsigned)sj < (unsigned)g.ws;
-            const size_t pix = (size_t)((pb[a] * g.hs + si) * g.ws + sj);
-            ra[a] = ok ? *reinterpret_cast<const uint4*>(src + pix * cs + c) : make_uint4(0, 0, 0, 0);
-        }
+        for (int u = 0; u < KC; ++u) {
+            const int gq = q * KC + u;
+            const bool live = gq < QG;
+            const int t = live ? gq / cpt : 0;
+            const int k = live ? (gq - t * cpt) * kGroup : 0;   // channel offset in [0, cin)
+            int ty, tx;
+            G::off(t, ty, tx);
+            const bool second = k >= g.c0;
+            const bf16* src = second ? g.src1 : g.src0;
+            const int cs = second ? g.c1 : g.c0;
+            const int c = (second ? k - g.c0 : k) + lq;
 #pragma unroll
-        for (int b = 0; b < PB; ++b) {
-            const int n = b * 64 + lrow;
-            if (n < BN) rb[b] = *reinterpret_cast<const uint4*>(g.w + (size_t)(bn0 + n) * g.K + t * cin + k + lq);
+            for (int a = 0; a < PA; ++a) {
+                const int si = pi[a] + ty, sj = pj[a] + tx;
+                const bool ok = live && pv[a] && (unsigned)si < (unsigned)g.hs && (unsigned)sj < (unsigned)g.ws;
+                const size_t pix = (size_t)((pb[a] * g.hs + si) * g.ws + sj);
+                ra[u][a] = ok ? *reinterpret_cast<const uint4*>(src + pix * cs + c) : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int b = 0; b < PB; ++b) {
+                const int n = b * 64 + lrow;
+                rb[u][b] = (live && n < BN)
+                               ? *reinterpret_cast<const uint4*>(g.w + (size_t)(bn0 + n) * g.K + t * cin + k + lq)
+                               : make_uint4(0, 0, 0, 0);
+            }
         }
     };
     auto swrite = [&](int buf) {
 #pragma unroll
-        for (int a = 0; a < PA; ++a) *reinterpret_cast<uint4*>(&sA[buf][(a * 64 + lrow) * kGemmLd + lq]) = ra[a];
+        for (int u = 0; u < KC; ++u) {
 #pragma unroll
-        for (int b = 0; b < PB; ++b) {
-            const int n = b * 64 + lrow;
-            if (n < BN) *reinterpret_cast<uint4*>(&sB[buf][n * kGemmLd + lq]) = rb[b];
+            for (int a = 0; a < PA; ++a)
+                *reinterpret_cast<uint4*>(&sA[buf][(a * 64 + lrow) * LD + u * kGroup + lq]) = ra[u][a];
+#pragma unroll
+            for (int b = 0; b < PB; ++b) {
+                const int n = b * 64 + lrow;
+                if (n < BN) *reinterpret_cast<uint4*>(&sB[buf][n * LD + u * kGroup + lq]) = rb[u][b];
+            }
         }
     };
 
@@ -169,19 +183,18 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
     for (int q = 0; q < Q; ++q) {
         const int buf = q & 1;
         if (q + 1 < Q) gload(q + 1);
+        // a zero-filled group past the end adds nothing
 #pragma unroll
-        for (int kk = 0; kk < kGemmBK / 16; ++kk) {
+        for (int kk = 0; kk < KC * kGroup / 16; ++kk) {
             bf16x8 fa[RM], fb[RN];
 #pragma unroll
             for (int a = 0; a < RM; ++a)
                 fa[a] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(
-                                &sA[buf][(wm * 32 * RM + a * 32 + r) * kGemmLd + kk * 16 + kh * 8]));
+                    bf16x8, *reinterpret_cast<const uint4*>(&sA[buf][(wm * 32 * RM + a * 32 + r) * LD + kk * 16 + kh * 8]));
 #pragma unroll
             for (int b = 0; b < RN; ++b)
                 fb[b] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(
-                                &sB[buf][(wn * 32 * RN + b * 32 + r) * kGemmLd + kk * 16 + kh * 8]));
+                    bf16x8, *reinterpret_cast<const uint4*>(&sB[buf][(wn * 32 * RN + b * 32 + r) * LD + kk * 16 + kh * 8]));
 #pragma unroll
             for (int a = 0; a < RM; ++a)
 #pragma unroll
@@ -525,21 +538,23 @@ __global__ void k_wt_transpose(const bf16* __restrict__ in, bf16* __restrict__ o
 
 static inline int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
-template <int TK, int WM, int WN, int RM, int RN>
+template <int TK, int WM, int WN, int RM, int RN, int KC>
 static void launch_gemm_t(const Gemm& g, hipStream_t s)
 {
     constexpr int BM = 32 * WM * RM, BN = 32 * WN * RN;
     const long long M = (long long)g.batch * g.hg * g.wg;
     dim3 grid(cdiv(M, BM), cdiv(g.N, BN));
-    hipLaunchKernelGGL((k_gemm<TK, WM, WN, RM, RN>), grid, dim3(256), 0, s, g);
+    hipLaunchKernelGGL((k_gemm<TK, WM, WN, RM, RN, KC>), grid, dim3(256), 0, s, g);
 }
 
 template <int TK>
 static void launch_gemm_tk(const Gemm& g, hipStream_t s)
 {
-    if (g.N % 128 == 0) launch_gemm_t<TK, 2, 2, 2, 2>(g, s);
-    else if (g.N % 64 == 0) launch_gemm_t<TK, 4, 1, 2, 2>(g, s);
-    else launch_gemm_t<TK, 4, 1, 2, 1>(g, s);
+    // two K groups per LDS stage for the 128 x 128 tiles; one for the
+    // 256-row tiles (registers and LDS)
+    if (g.N % 128 == 0) launch_gemm_t<TK, 2, 2, 2, 2, 2>(g, s);
+    else if (g.N % 64 == 0) launch_gemm_t<TK, 4, 1, 2, 2, 1>(g, s);
+    else launch_gemm_t<TK, 4, 1, 2, 1, 1>(g, s);
 }
 
 // The tap pattern is given by ntaps, stride and the first offset; the
