@@ -380,13 +380,16 @@ def error_stats(pred, truth):
             "arrival_mae_h": float(np.abs(pred[:, 1] - truth[:, 1]).mean())}
 
 
+INFER_BATCH = 64
+
+
 def screen(net, scs, X, heldout_frac=0.05, seed=1, chunk=256, log=None):
     """Surrogate screening of the scenarios (features X): predict every
     scenario, send to the solver those whose predicted asset depth lies
     within MARGIN of THRESHOLD plus a seeded held-out share, and judge the
     surrogate's error on the held-out scenarios.  Returns the decisions,
     the final maps (solver maps where the solver ran) and the timings."""
-    pred, t_inf = predict(net, X)
+    pred, t_inf = predict(net, X, min(net.max_batch, INFER_BATCH))
     q = asset_depth(pred[:, 0])
     near = np.abs(q - THRESHOLD) < MARGIN
     rng = np.random.default_rng(seed)
@@ -440,9 +443,13 @@ def run(n_train=4000, n_test=2048, epochs=20, batch=16, lr=1e-3, train_seed0=500
     keep = ~np.isnan(Ytr).any(axis=(1, 2, 3))      # drop runs that blew up
     Xtr, Ytr = Xtr[keep], Ytr[keep]
     side = scs_te[0]["side"]
-    net = UNet(side, side, len(FEATURES), 2, 32, 4, max_batch=batch, seed=1, device=device)
+    # trained at `batch`, run for inference at batch 64 (more blocks for the
+    # small levels)
+    net = UNet(side, side, len(FEATURES), 2, 32, 4, max_batch=max(batch, INFER_BATCH), seed=1, device=device)
     net.set_loss_weights([1.0, 0.25])
-    fwd_flops, train_flops = net.flops(batch)
+    max_b = net.max_batch
+    fwd_flops, _ = net.flops(max_b)
+    _, train_flops = net.flops(batch)
     hist, train_s = train(net, Xtr, Ytr, epochs, batch, lr, seed=2, log=log)
     del Xtr, Ytr
     res = screen(net, scs_te, Xte, heldout_frac=heldout, seed=3, chunk=chunk, log=log)
@@ -456,7 +463,7 @@ def run(n_train=4000, n_test=2048, epochs=20, batch=16, lr=1e-3, train_seed0=500
         "train_solver_s": st_tr["setup_s"] + st_tr["run_s"] + st_tr["copy_s"],
         "train_s": train_s, "train_loss": hist,
         "train_tflops": train_flops * epochs * (n_train // batch) / max(train_s, 1e-9) / 1e12,
-        "infer_s": res["infer_s"], "infer_tflops": fwd_flops / batch * n_test / max(res["infer_s"], 1e-9) / 1e12,
+        "infer_s": res["infer_s"], "infer_tflops": fwd_flops / max_b * n_test / max(res["infer_s"], 1e-9) / 1e12,
         "train_solver_failed": len(st_tr["failed"]), "screen_solver_failed": len(st["failed"]),
         "fallback": int(res["fallback"].sum()), "heldout": int(res["heldout"].sum()), "solver_runs": runs,
         "screened_s": screened_s, "screened_per_hour": n_test / screened_s * 3600.0,
