@@ -255,7 +255,9 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 // A the same tap gather as k_gemm and G the output gradient read through the
 // same output mapping.  Block tile 32 (n) x 128 (k), one 32 x 32 tile per
 // wave; pixels in chunks of 64, both operands staged transposed in LDS
-// (rows of 64 pixels, padded to 72).  Partial sums go to dW with FP32
+// (rows of 64 pixels, padded to 72, columns swizzled per row group).  A
+// variant storing packed pixel pairs (half the LDS instructions) ran slower.
+// Partial sums go to dW with FP32
 // atomics, so the summation order varies between runs.  The blocks of the
 // first K tile also sum the staged output gradient over pixels: the bias
 // gradient (for a transposed conv, every sub-pixel adds to its channel).
@@ -271,6 +273,10 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
     const int hw = w.hg * w.wg, M = w.batch * hw;
     const int cin = w.c0 + w.c1;
     const int px = tid >> 2, cv = tid & 3;
+    // pixel columns of LDS row R are XOR-swizzled by 16 x ((R / 8) mod 4):
+    // the four channel octets a wave stores for one pixel land in different
+    // banks, and 8-pixel fragments stay contiguous
+    const int swz_w = 16 * cv, swz_r = 16 * (((threadIdx.x & 31) >> 3) & 3);
     // G column group: the 32 columns share one sub-pixel (cout % 32 == 0)
     const int sub = n0 / w.cout, co0 = n0 - sub * w.cout;
     const int su = sub / w.ups, sv = sub - su * w.ups;
@@ -311,7 +317,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
             const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
             for (int e = 0; e < 8; ++e)
-                sG[(cv * 8 + e) * kWgPad + px] = (bf16)(wv[e >> 1] >> ((e & 1) * 16));
+                sG[(cv * 8 + e) * kWgPad + (px ^ swz_w)] = (bf16)(wv[e >> 1] >> ((e & 1) * 16));
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
             const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
 #pragma unroll
             for (int e = 0; e < 8; ++e)
-                sA[(q * 32 + cv * 8 + e) * kWgPad + px] = (bf16)(wv[e >> 1] >> ((e & 1) * 16));
+                sA[(q * 32 + cv * 8 + e) * kWgPad + (px ^ swz_w)] = (bf16)(wv[e >> 1] >> ((e & 1) * 16));
         }
         __syncthreads();
         if (bias) {
@@ -341,9 +347,10 @@ __global__ __launch_bounds__(256) void k_wgrad(WGrad w)
 #pragma unroll
             for (int st = 0; st < kWgPix / 16; ++st) {
                 const int c = st * 16 + kh * 8;
-                const bf16x8 fa = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sG[r * kWgPad + c]));
+                const int cs = c ^ swz_r;
+                const bf16x8 fa = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sG[r * kWgPad + cs]));
                 const bf16x8 fb =
-                    __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sA[(wave * 32 + r) * kWgPad + c]));
+                    __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sA[(wave * 32 + r) * kWgPad + cs]));
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc, 0, 0, 0);
             }
         }
