@@ -82,17 +82,27 @@ struct TapGeo {
 // K advances in groups of 32 = one tap of one 32-channel block of one
 // source; a stage holds KC groups.  A stage's A tile (BM pixels x 32 KC) and
 // B tile (BN columns x 32 KC) are loaded from global memory into registers
-// one stage ahead, stored to LDS (double-buffered, rows padded by 8 elements
-// so that the 16-byte fragment reads of 8 consecutive rows hit disjoint
-// banks), and the waves read their MFMA fragments from LDS.
+// one stage ahead, stored to LDS (double-buffered), and the waves read their
+// MFMA fragments from LDS.  The 16-byte chunks of an LDS row are
+// XOR-swizzled by the row (gemm_chunk): the fragment reads of 8 consecutive
+// rows and the stores of 2 consecutive rows each cover all 32 banks once.
 constexpr int kGroup = 32;
+
+// Element offset of 16-byte chunk c of LDS row r, rows of KC * 32 elements.
+template <int KC>
+__device__ __forceinline__ int gemm_chunk(int r, int c)
+{
+    if constexpr (KC == 1) return 8 * (c ^ ((r >> 1) & 3));
+    else return 8 * (c ^ (((r & 1) << 2) | ((r >> 1) & 3)));
+}
 
 template <int TK, int WM, int WN, int RM, int RN, int KC>
 __global__ __launch_bounds__(256) void k_gemm(Gemm g)
 {
     using G = TapGeo<TK>;
     constexpr int BM = 32 * WM * RM, BN = 32 * WN * RN;
-    constexpr int LD = KC * kGroup + 8;         // LDS row stride (elements)
+    static_assert(KC == 1 || KC == 2, "one or two K groups per stage");
+    constexpr int LD = KC * kGroup;             // LDS row stride (elements)
     constexpr int PA = BM / 64;                 // A rows per thread and group
     constexpr int PB = (BN + 63) / 64;          // B rows per thread and group
     __shared__ __attribute__((aligned(16))) bf16 sA[2][BM * LD];
@@ -158,13 +168,16 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
     auto swrite = [&](int buf) {
 #pragma unroll
         for (int u = 0; u < KC; ++u) {
+            const int c = u * 4 + (lq >> 3);
 #pragma unroll
-            for (int a = 0; a < PA; ++a)
-                *reinterpret_cast<uint4*>(&sA[buf][(a * 64 + lrow) * LD + u * kGroup + lq]) = ra[u][a];
+            for (int a = 0; a < PA; ++a) {
+                const int row = a * 64 + lrow;
+                *reinterpret_cast<uint4*>(&sA[buf][row * LD + gemm_chunk<KC>(row, c)]) = ra[u][a];
+            }
 #pragma unroll
             for (int b = 0; b < PB; ++b) {
                 const int n = b * 64 + lrow;
-                if (n < BN) *reinterpret_cast<uint4*>(&sB[buf][n * LD + u * kGroup + lq]) = rb[u][b];
+                if (n < BN) *reinterpret_cast<uint4*>(&sB[buf][n * LD + gemm_chunk<KC>(n, c)]) = rb[u][b];
             }
         }
     };
@@ -188,13 +201,17 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
         for (int kk = 0; kk < KC * kGroup / 16; ++kk) {
             bf16x8 fa[RM], fb[RN];
 #pragma unroll
-            for (int a = 0; a < RM; ++a)
-                fa[a] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(&sA[buf][(wm * 32 * RM + a * 32 + r) * LD + kk * 16 + kh * 8]));
+            for (int a = 0; a < RM; ++a) {
+                const int row = wm * 32 * RM + a * 32 + r;
+                fa[a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
+                                                       &sA[buf][row * LD + gemm_chunk<KC>(row, 2 * kk + kh)]));
+            }
 #pragma unroll
-            for (int b = 0; b < RN; ++b)
-                fb[b] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(&sB[buf][(wn * 32 * RN + b * 32 + r) * LD + kk * 16 + kh * 8]));
+            for (int b = 0; b < RN; ++b) {
+                const int row = wn * 32 * RN + b * 32 + r;
+                fb[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(
+                                                       &sB[buf][row * LD + gemm_chunk<KC>(row, 2 * kk + kh)]));
+            }
 #pragma unroll
             for (int a = 0; a < RM; ++a)
 #pragma unroll
