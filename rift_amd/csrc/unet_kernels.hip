@@ -237,24 +237,35 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
             dstride = g.cout - g.dsplit;
             dco = co - g.dsplit;
         }
+        // a lane's rows come in runs of 4 consecutive pixels starting at a
+        // multiple of 4: with wg % 4 == 0 they share image and grid row, so
+        // the pixel is decomposed once per run
+        const bool quads = (g.wg & 3) == 0;
 #pragma unroll
         for (int a = 0; a < RM; ++a)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < 4; ++q) {
+                const int m4 = m0 + a * 32 + 8 * q + 4 * kh;
+                int ib4 = 0, rem4 = m4, i4 = 0, j4 = 0;
+                if (quads && (g.ups != 1 || g.dstf)) {
+                    ib4 = m4 / hw;
+                    rem4 = m4 - ib4 * hw;
+                    i4 = rem4 / g.wg;
+                    j4 = rem4 - i4 * g.wg;
+                }
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const int m = m0 + a * 32 + 8 * q + 4 * kh + e;
+                    const int m = m4 + e;
                     if (m >= M) continue;
                     long long op = m;           // output pixel (same grid when ups == 1)
-                    int ib = 0, rem = m;
-                    if (g.ups != 1 || g.dstf) {
+                    int ib = ib4, rem = rem4 + e, i = i4, j = j4 + e;
+                    if (!quads && (g.ups != 1 || g.dstf)) {
                         ib = m / hw;
                         rem = m - ib * hw;
+                        i = rem / g.wg;
+                        j = rem - i * g.wg;
                     }
-                    if (g.ups != 1) {
-                        const int i = rem / g.wg, j = rem - i * g.wg;
-                        op = ((long long)ib * g.ho + (g.ups * i + su)) * g.wo + (g.ups * j + sv);
-                    }
+                    if (g.ups != 1) op = ((long long)ib * g.ho + (g.ups * i + su)) * g.wo + (g.ups * j + sv);
                     float v = acc[a][b][4 * q + e] + bias;
                     if (g.relu) v = fmaxf(v, 0.f);
                     if (g.mask && !(bf2f(g.mask[op * g.cout + co]) > 0.f)) v = 0.f;
@@ -264,6 +275,7 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g)
                         dst[op * dstride + dco] = f2bf(v);
                     }
                 }
+            }
     }
 }
 
