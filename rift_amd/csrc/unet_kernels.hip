@@ -484,17 +484,18 @@ __global__ void k_to_nhwc(const float* __restrict__ x, bf16* __restrict__ y, int
 
 // Weighted mean-square loss of planar y against planar target; writes the
 // loss gradient NHWC bfloat16 [P][cpad] (zero beyond C) and adds the loss to
-// *loss in FP64.
+// *loss in FP64 (one atomic per block).
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ tgt,
                                             const float* __restrict__ wts, int B, int C, int H, int W, int cpad,
                                             bf16* __restrict__ dy, double* __restrict__ loss)
 {
     __shared__ double part[4];
     const long long hw = (long long)H * W, n = (long long)B * hw;
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const double inv = 1.0 / (double)n;
     double l = 0.0;
-    if (p < n) {
+    // grid-stride, so that few blocks add to the one FP64 accumulator
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (long long)gridDim.x * blockDim.x) {
         const long long b = p / hw, s = p - b * hw;
         for (int c0 = 0; c0 < cpad; c0 += 8) {
             uint32_t o[4];
@@ -640,7 +641,8 @@ void launch_mse(const float* y, const float* tgt, const float* wts, int B, int C
                 double* loss, hipStream_t s)
 {
     const long long n = (long long)B * H * W;
-    hipLaunchKernelGGL(k_mse, dim3(cdiv(n, 256)), dim3(256), 0, s, y, tgt, wts, B, C, H, W, cpad, dy, loss);
+    hipLaunchKernelGGL(k_mse, dim3(std::min(cdiv(n, 256), 512)), dim3(256), 0, s, y, tgt, wts, B, C, H, W, cpad, dy,
+                       loss);
 }
 
 void launch_adam(float* p, float* g, float* m, float* v, bf16* pb, long long n, float lr, float b1, float b2,
