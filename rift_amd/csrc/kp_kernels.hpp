@@ -185,7 +185,7 @@ void launch_grow2(const Dev& d, hipStream_t s);
 void launch_timestep2(const Dev& d, hipStream_t s);
 void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks);
 // workgroups of a marching kernel resident at once on a device of `cus` CUs
-int march_resident_blocks(int stage, int variant, bool textbook, bool budget, int cus);
+int march_resident_blocks(int stage, int variant, unsigned flags, int cus);
 constexpr int kMarchVariants = 4;
 // Ensembles (devs: device array of the members' Dev)
 void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s);

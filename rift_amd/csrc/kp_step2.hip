@@ -197,7 +197,15 @@ __device__ __forceinline__ size_t row_of(const Dev& d, int j)
     return (size_t)j * (size_t)d.P;
 }
 
-template <int Stage>
+// Feature flags an instantiation of the marching path may see.  The lean
+// kernels are compiled without gridded forcing, point sources and summary
+// maps, so that the pointers and branches of those features take no
+// registers in the common case.
+constexpr unsigned kLeanMask = ~(RIFT_RAIN_GRID | RIFT_DRAIN_GRID | RIFT_SURGE_GRID | RIFT_HYDROGRAPH |
+                                 RIFT_SAVE_MAX | RIFT_SAVE_ARRIVAL);
+constexpr unsigned kFullMask = ~0u;
+
+template <int Stage, unsigned FM>
 __device__ __forceinline__ Raw fetch(const Dev& d, const Lane& L, int j)
 {
     const int sr = min(max(j, 2), d.ny - 3);
@@ -214,12 +222,13 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const Lane& L, int j)
         r.dh = ld(d.dh + rb, L.sc4);
         r.dhu = ld(d.dhu + rb, L.sc4);
         r.dhv = ld(d.dhv + rb, L.sc4);
-        r.sg = (d.flags & RIFT_SURGE_GRID) ? ld(d.map[RIFT_MAP_SURGE][0] + rb, L.sc4) : 0.f;
+        r.sg = (d.flags & FM & RIFT_SURGE_GRID) ? ld(d.map[RIFT_MAP_SURGE][0] + rb, L.sc4) : 0.f;
         r.n = d.n_uniform ? d.n_const : ld(d.n + rb, L.sc4);
     }
     return r;
 }
 
+template <unsigned FM>
 __device__ __forceinline__ Src fetch_src(const Dev& d, const Lane& L, int j)
 {
     const size_t rb = row_of(d, j);
@@ -227,12 +236,12 @@ __device__ __forceinline__ Src fetch_src(const Dev& d, const Lane& L, int j)
     s.bx = ld(d.BX + rb, L.c4);
     s.K = (d.flags & RIFT_INFILTRATION) ? ld(d.K + rb, L.c4) : 0.f;
     s.F = (d.flags & RIFT_INFILTRATION) ? ld(d.F + rb, L.c4) : 0.f;
-    s.rain = (d.flags & RIFT_RAIN_GRID) ? ld(d.map[RIFT_MAP_RAIN][0] + rb, L.c4) : 0.f;
-    s.drain = (d.flags & RIFT_DRAIN_GRID) ? ld(d.map[RIFT_MAP_DRAIN][0] + rb, L.c4) : 0.f;
+    s.rain = (d.flags & FM & RIFT_RAIN_GRID) ? ld(d.map[RIFT_MAP_RAIN][0] + rb, L.c4) : 0.f;
+    s.drain = (d.flags & FM & RIFT_DRAIN_GRID) ? ld(d.map[RIFT_MAP_DRAIN][0] + rb, L.c4) : 0.f;
     return s;
 }
 
-template <int Stage>
+template <int Stage, unsigned FM>
 __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& raw, int j, float dt,
                                       float inv_dt)
 {
@@ -252,7 +261,7 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
             const float G = manning_g(raw.n, hu, hv, h, d.kappa);
             const float hcl = (h > kEps) ? h : 0.f;
             float dh = raw.dh;
-            if ((d.flags & RIFT_SURGE_GRID) && raw.sg != d.nodata && raw.sg > 0.f)
+            if ((d.flags & FM & RIFT_SURGE_GRID) && raw.sg != d.nodata && raw.sg > 0.f)
                 dh = (raw.sg - fmaf(dh, dt, hcl)) * inv_dt;
             const float hnew = fmaf(dh, dt, hcl);
             const float rden = frcp(fmaf(-dt, G, 1.f));
@@ -367,13 +376,13 @@ struct BudgetAcc {
 // (fE, fw, gn, gs: mass fluxes of its east, west, north, south faces), and in
 // stage 2 the gridded rain, the drain of both stages and the exchange of the
 // dry clamps and the surge override of this step.
-template <int Stage, bool Budget>
+template <int Stage, bool Budget, unsigned FM>
 __device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, const Lane& L, int c,
                                            int j, const Row& C, const Src& src, float dh, float dhu,
                                            float dhv, float bxw, float bxe, float bys, float byn,
                                            float fE, float fw, float gn, float gs, BudgetAcc& acc)
 {
-    const unsigned fl = S.fl;
+    const unsigned fl = S.fl & FM;
     const size_t rbo = row_of(d, j);   // output row (uniform)
     const float hC = C.h;
     const float S1 = -kG * hC * ((bxe - bxw) * S.inv_dx);
@@ -492,7 +501,7 @@ __device__ __forceinline__ void wave_add_f64(double* dst, float v, int lane)
     if (lane == 0 && x != 0.0) atomicAdd(dst, x);
 }
 
-template <int Stage, bool Textbook, bool Up, bool Budget>
+template <int Stage, bool Textbook, bool Up, bool Budget, unsigned FM>
 __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane)
 {
     constexpr int step = Up ? -1 : 1;
@@ -506,15 +515,15 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
 
     // prologue: rows j-2 .. j+2 (both directions), by = BY of a row = the bed
     // of its south face
-    const Row A = finish<Stage>(d, L, fetch<Stage>(d, L, j - 2 * step), j - 2 * step, dt, inv_dt);
-    const Raw rB = fetch<Stage>(d, L, j - step);
-    const Raw rC = fetch<Stage>(d, L, j);
-    const Raw rD = fetch<Stage>(d, L, j + step);
-    Raw rE = fetch<Stage>(d, L, j + 2 * step);
-    Src src = fetch_src(d, L, j);
-    const Row B = finish<Stage>(d, L, rB, j - step, dt, inv_dt);
-    Row C = finish<Stage>(d, L, rC, j, dt, inv_dt);
-    Row D = finish<Stage>(d, L, rD, j + step, dt, inv_dt);
+    const Row A = finish<Stage, FM>(d, L, fetch<Stage, FM>(d, L, j - 2 * step), j - 2 * step, dt, inv_dt);
+    const Raw rB = fetch<Stage, FM>(d, L, j - step);
+    const Raw rC = fetch<Stage, FM>(d, L, j);
+    const Raw rD = fetch<Stage, FM>(d, L, j + step);
+    Raw rE = fetch<Stage, FM>(d, L, j + 2 * step);
+    Src src = fetch_src<FM>(d, L, j);
+    const Row B = finish<Stage, FM>(d, L, rB, j - step, dt, inv_dt);
+    Row C = finish<Stage, FM>(d, L, rC, j, dt, inv_dt);
+    Row D = finish<Stage, FM>(d, L, rD, j + step, dt, inv_dt);
     FaceStates yC;
     Flux gB;          // flux of the face between B and C
     float byC = rC.by, byD = rD.by, byB = rB.by;
@@ -550,10 +559,10 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         // prefetch: the row after next and the next output row's inputs
         // (clamped to the domain; unused past the end of the run)
         const int jn = Up ? max(j - 3, 0) : j + 3;
-        const Raw rN = fetch<Stage>(d, L, jn);
-        const Src srcN = fetch_src(d, L, min(max(j + step, 2), d.ny - 3));
+        const Raw rN = fetch<Stage, FM>(d, L, jn);
+        const Src srcN = fetch_src<FM>(d, L, min(max(j + step, 2), d.ny - 3));
 
-        const Row E = finish<Stage>(d, L, rE, j + 2 * step, dt, inv_dt);
+        const Row E = finish<Stage, FM>(d, L, rE, j + 2 * step, dt, inv_dt);
         const float byE = rE.by;
         FaceStates yD;
         Flux gD;      // flux of the face between C and D
@@ -589,7 +598,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             myl = fmaxf(myl, gn.speed);
             // beds of the south / north faces of row j
             const float bys = byC, byn = Up ? byB : byD;
-            wetl |= output_row<Stage, Budget>(d, S, L, c, j, C, src, dh, dhu, dhv, bxw, bxe, bys, byn,
+            wetl |= output_row<Stage, Budget, FM>(d, S, L, c, j, C, src, dh, dhu, dhv, bxw, bxe, bys, byn,
                                               fE.f0, fw0, gn.f0, gs.f0, acc);
         }
         // roll the window
@@ -611,8 +620,8 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         } else {
             wave_add_f64(&sc->b_out, acc.out, lane);
             wave_add_f64(&sc->b_exch, acc.exch, lane);
-            if (S.fl & RIFT_RAIN_GRID) wave_add_f64(&sc->b_rain_grid, acc.rain, lane);
-            if (S.fl & (RIFT_DRAIN_AVG | RIFT_DRAIN_GRID)) wave_add_f64(&sc->b_drain, acc.drain, lane);
+            if (S.fl & FM & RIFT_RAIN_GRID) wave_add_f64(&sc->b_rain_grid, acc.rain, lane);
+            if (S.fl & FM & (RIFT_DRAIN_AVG | RIFT_DRAIN_GRID)) wave_add_f64(&sc->b_drain, acc.drain, lane);
         }
     }
 }
@@ -634,7 +643,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
 // Otherwise: the active units (strip-major) form a sequence of rows and wave
 // i marches the i-th equal share of at least Dev::min_rows rows, downwards,
 // continuing from one unit into the next while they are vertically adjacent.
-template <int Stage, bool Textbook, int Waves, bool Budget>
+template <int Stage, bool Textbook, int Waves, bool Budget, bool Lean>
 __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
 {
     const int lane = threadIdx.x & 63;
@@ -678,8 +687,8 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         const int ja = 2 + (int)((long long)rows * mi / m);
         const int jb = 2 + (int)((long long)rows * (mi + 1) / m);
         if (ja >= jb) return;
-        if (mi & 1) march_run<Stage, Textbook, true, Budget>(d, S, s, ja, jb, lane);
-        else march_run<Stage, Textbook, false, Budget>(d, S, s, ja, jb, lane);
+        if (mi & 1) march_run<Stage, Textbook, true, Budget, Lean ? kLeanMask : kFullMask>(d, S, s, ja, jb, lane);
+        else march_run<Stage, Textbook, false, Budget, Lean ? kLeanMask : kFullMask>(d, S, s, ja, jb, lane);
         return;
     }
 
@@ -708,7 +717,7 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         const int rlast = last - s * d.nurows;
         const long long pe = min(pend, (long long)(kk + 1) * kTileH);
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (long long)kk * kTileH), d.ny - 2);
-        if (ja < jb) march_run<Stage, Textbook, false, Budget>(d, S, s, ja, jb, lane);
+        if (ja < jb) march_run<Stage, Textbook, false, Budget, Lean ? kLeanMask : kFullMask>(d, S, s, ja, jb, lane);
         k = kk + 1;
         a = 0;
         if ((long long)k * kTileH >= pend) break;
@@ -874,14 +883,29 @@ void launch_grow2(const Dev& d, hipStream_t s)
 // bit 1 = stage 2 at 3 instead of 4 waves per SIMD; the dealing of work
 // shares and their least size are Dev::band and Dev::min_rows.  The grid is
 // the number of workgroups resident at once (march_resident_blocks).
+// The lean instantiation serves the non-textbook domains that use none of
+// the features outside kLeanMask.
+bool march_lean(unsigned flags)
+{
+    return !(flags & RIFT_TEXTBOOK) && (flags & ~kLeanMask) == 0u;
+}
+
 template <int Stage, int Waves>
 static void launch_march_w(const Dev& d, int blocks, hipStream_t s)
 {
     const bool tb = d.flags & RIFT_TEXTBOOK, bud = d.flags & RIFT_MASS_BUDGET;
-    if (tb && bud) hipLaunchKernelGGL((k_march<Stage, true, Waves, true>), dim3(blocks), dim3(256), 0, s, d);
-    else if (tb) hipLaunchKernelGGL((k_march<Stage, true, Waves, false>), dim3(blocks), dim3(256), 0, s, d);
-    else if (bud) hipLaunchKernelGGL((k_march<Stage, false, Waves, true>), dim3(blocks), dim3(256), 0, s, d);
-    else hipLaunchKernelGGL((k_march<Stage, false, Waves, false>), dim3(blocks), dim3(256), 0, s, d);
+    if (march_lean(d.flags)) {
+        if (bud) hipLaunchKernelGGL((k_march<Stage, false, Waves, true, true>), dim3(blocks), dim3(256), 0, s, d);
+        else hipLaunchKernelGGL((k_march<Stage, false, Waves, false, true>), dim3(blocks), dim3(256), 0, s, d);
+    } else if (tb && bud) {
+        hipLaunchKernelGGL((k_march<Stage, true, Waves, true, false>), dim3(blocks), dim3(256), 0, s, d);
+    } else if (tb) {
+        hipLaunchKernelGGL((k_march<Stage, true, Waves, false, false>), dim3(blocks), dim3(256), 0, s, d);
+    } else if (bud) {
+        hipLaunchKernelGGL((k_march<Stage, false, Waves, true, false>), dim3(blocks), dim3(256), 0, s, d);
+    } else {
+        hipLaunchKernelGGL((k_march<Stage, false, Waves, false, false>), dim3(blocks), dim3(256), 0, s, d);
+    }
 }
 
 void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks)
@@ -897,23 +921,28 @@ void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int block
 }
 
 template <int Stage, int Waves>
-static int resident(bool textbook, bool budget, int cus)
+static int resident(unsigned flags, int cus)
 {
-    const void* k = textbook ? (budget ? (const void*)k_march<Stage, true, Waves, true>
-                                       : (const void*)k_march<Stage, true, Waves, false>)
-                             : (budget ? (const void*)k_march<Stage, false, Waves, true>
-                                       : (const void*)k_march<Stage, false, Waves, false>);
+    const bool textbook = flags & RIFT_TEXTBOOK, budget = flags & RIFT_MASS_BUDGET;
+    const void* k;
+    if (march_lean(flags))
+        k = budget ? (const void*)k_march<Stage, false, Waves, true, true>
+                   : (const void*)k_march<Stage, false, Waves, false, true>;
+    else
+        k = textbook ? (budget ? (const void*)k_march<Stage, true, Waves, true, false>
+                               : (const void*)k_march<Stage, true, Waves, false, false>)
+                     : (budget ? (const void*)k_march<Stage, false, Waves, true, false>
+                               : (const void*)k_march<Stage, false, Waves, false, false>);
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     return per_cu * cus;
 }
 
-int march_resident_blocks(int stage, int variant, bool textbook, bool budget, int cus)
+int march_resident_blocks(int stage, int variant, unsigned flags, int cus)
 {
-    if (stage == 1)
-        return (variant & 1) ? resident<1, 4>(textbook, budget, cus) : resident<1, 5>(textbook, budget, cus);
-    return (variant & 2) ? resident<2, 3>(textbook, budget, cus) : resident<2, 4>(textbook, budget, cus);
+    if (stage == 1) return (variant & 1) ? resident<1, 4>(flags, cus) : resident<1, 5>(flags, cus);
+    return (variant & 2) ? resident<2, 3>(flags, cus) : resident<2, 4>(flags, cus);
 }
 
 void launch_canonicalize(const Dev& d, hipStream_t s)
@@ -1063,7 +1092,7 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
         const int rlast = last - s * d.nurows;
         const long long pe = min(pend, (base + kk + 1) * kTileH);
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (base + kk) * kTileH), d.ny - 2);
-        if (ja < jb) march_run<Stage, Textbook, false, Budget>(d, S, s, ja, jb, lane);
+        if (ja < jb) march_run<Stage, Textbook, false, Budget, kFullMask>(d, S, s, ja, jb, lane);
         k = base + kk + 1;
         a = 0;
         if (k * kTileH >= pend) break;

@@ -300,9 +300,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
         g->cus = prop.multiProcessorCount;
         for (int st = 0; st < 2; ++st)
             for (int v = 0; v < kMarchVariants; ++v)
-                g->march_blocks[st][v] =
-                    march_resident_blocks(st + 1, v, (desc->flags & RIFT_TEXTBOOK) != 0,
-                                          (desc->flags & RIFT_MASS_BUDGET) != 0, g->cus);
+                g->march_blocks[st][v] = march_resident_blocks(st + 1, v, desc->flags, g->cus);
     }
     *out = g;
     return RIFT_OK;
