@@ -157,7 +157,11 @@ struct Src {                 // own-cell inputs of an output row
 };
 
 // Per-lane constants of one unit.  The rows of a unit touch tile rows r-1,
-// r, r+1 only; their buffer selects and activity flags are read once.
+// r, r+1 only; their buffer selects and activity flags are read once, and
+// those of the tile row after r+1 in marching direction (r+2 down, r-2 up)
+// are loaded one tile row ahead: the loads retire in order, so a flag loaded
+// at the crossing itself would make the wave wait for every row prefetch in
+// flight.
 // Cells are addressed as a row pointer (wave-uniform, scalar registers) plus
 // a 32-bit byte offset per lane (the column, and one plane more for a tile
 // whose state lives in the second buffer), which the global loads and stores
@@ -169,6 +173,7 @@ struct Lane {
     bool gx;
     bool sel0, sel1, sel2;   // buffer selects of tile rows r-1, r, r+1
     bool act0, act1, act2;   // activity of tile rows r-1, r, r+1
+    bool selN, actN;         // the same for tile row r+2 (down) or r-2 (up)
 };
 
 __device__ __forceinline__ float ld(const float* row, unsigned byte_off)
@@ -284,6 +289,7 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
 }
 
 // Lane constants for strip s, tile row rr.
+template <bool Up>
 __device__ __forceinline__ Lane make_lane(const Dev& d, int s, int rr, int lane)
 {
     Lane L;
@@ -299,6 +305,9 @@ __device__ __forceinline__ Lane make_lane(const Dev& d, int s, int rr, int lane)
     const int Tp = min(rr + 1, d.nby - 1) * d.nbx + L.txo;
     L.sel0 = d.sel[Tm] != 0; L.sel1 = d.sel[T0] != 0; L.sel2 = d.sel[Tp] != 0;
     L.act0 = d.act[Tm] != 0; L.act1 = d.act[T0] != 0; L.act2 = d.act[Tp] != 0;
+    const int TN = (Up ? max(rr - 2, 0) : min(rr + 2, d.nby - 1)) * d.nbx + L.txo;
+    L.selN = d.sel[TN] != 0;
+    L.actN = d.act[TN] != 0;
     return L;
 }
 
@@ -308,18 +317,15 @@ __device__ __forceinline__ void shift_lane(const Dev& d, Lane& L, int rr)
 {
     L.r = rr;
     if constexpr (!Up) {
-        L.sel0 = L.sel1; L.sel1 = L.sel2;
-        L.act0 = L.act1; L.act1 = L.act2;
-        const int Tp = min(rr + 1, d.nby - 1) * d.nbx + L.txo;
-        L.sel2 = d.sel[Tp] != 0;
-        L.act2 = d.act[Tp] != 0;
+        L.sel0 = L.sel1; L.sel1 = L.sel2; L.sel2 = L.selN;
+        L.act0 = L.act1; L.act1 = L.act2; L.act2 = L.actN;
     } else {
-        L.sel2 = L.sel1; L.sel1 = L.sel0;
-        L.act2 = L.act1; L.act1 = L.act0;
-        const int Tm = max(rr - 1, 0) * d.nbx + L.txo;
-        L.sel0 = d.sel[Tm] != 0;
-        L.act0 = d.act[Tm] != 0;
+        L.sel2 = L.sel1; L.sel1 = L.sel0; L.sel0 = L.selN;
+        L.act2 = L.act1; L.act1 = L.act0; L.act0 = L.actN;
     }
+    const int TN = (Up ? max(rr - 2, 0) : min(rr + 2, d.nby - 1)) * d.nbx + L.txo;
+    L.selN = d.sel[TN] != 0;
+    L.actN = d.act[TN] != 0;
 }
 
 // Scalars of a marching launch.
@@ -345,8 +351,10 @@ __device__ __forceinline__ void flush_tiles(const Dev& d, int s, int rr, const L
     strip_tiles(d, s, txa, txb);
     for (int tx = txa; tx <= txb; ++tx) {
         const int T = rr * d.nbx + tx;
-        if (!d.act[T]) continue;   // uniform
+        // every tile of the strip holds output lanes, whose L.act1 is the
+        // tile's activity: no load (and no wait on the row prefetches)
         const bool seg = out_lane && L.txo == tx;
+        if (__ballot(seg && L.act1) == 0ull) continue;   // uniform
         const float m1 = wave_max(seg ? mxl : 0.f);
         const float m2 = wave_max(seg ? myl : 0.f);
         const bool anyw = __ballot(seg && wetl) != 0ull;
@@ -487,12 +495,6 @@ __device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, c
     }
 }
 
-// One run of a wave: output rows [ja, jb) of strip s, marched downwards
-// (increasing j) or upwards.  The window holds the output row C, the next
-// row D (in marching direction) and the raw row after it; the y flux of the
-// face behind C is carried from the previous row.  Crossing into another
-// tile row flushes the per-tile results and shifts the lane's tile-row
-// window.
 // FP64 wave sum of one lane value, added to *dst by lane 0.
 __device__ __forceinline__ void wave_add_f64(double* dst, float v, int lane)
 {
@@ -501,6 +503,29 @@ __device__ __forceinline__ void wave_add_f64(double* dst, float v, int lane)
     if (lane == 0 && x != 0.0) atomicAdd(dst, x);
 }
 
+template <int K>
+struct Phase {
+    static constexpr int value = K;
+};
+
+// Loop form: the row body instantiated once per window phase (register
+// renaming) or once with the window advanced by copies.  Three copies of the
+// body lengthen live ranges: the lean kernels have the registers for it, the
+// full and ensemble kernels (more pointers and flags live) would spill.
+template <unsigned FM>
+constexpr bool phased_rows() { return FM == kLeanMask; }
+
+// One run of a wave: output rows [ja, jb) of strip s, marched downwards
+// (increasing j) or upwards.  The window holds the output row C, the next
+// row D (in marching direction) and the raw row after it; the y flux of the
+// face behind C is carried from the previous row.  Crossing into another
+// tile row flushes the per-tile results and shifts the lane's tile-row
+// window.
+//
+// The window lives in three slots per quantity: row j + m*step of output row
+// j sits in slot (k + m) mod 3, where k is the phase of row j.  The loop body
+// is instantiated once per phase, so that advancing the window renames
+// registers at compile time instead of copying ~40 values per row.
 template <int Stage, bool Textbook, bool Up, bool Budget, unsigned FM>
 __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane)
 {
@@ -510,43 +535,56 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
     int j = Up ? jb - 1 : ja;
     const int jstop = Up ? ja - 1 : jb;
     int rr = (j - 2) / kTileH;
-    Lane L = make_lane(d, s, rr, lane);
+    Lane L = make_lane<Up>(d, s, rr, lane);
     const float dt = S.dt, inv_dt = S.inv_dt, kappa = S.kappa;
 
-    // prologue: rows j-2 .. j+2 (both directions), by = BY of a row = the bed
-    // of its south face
+    Row R[3];          // finished rows C, D (and E once formed)
+    Raw W[3];          // raw rows in flight
+    Src SR[3];         // own-cell inputs of the output row and the next
+    FaceStates Y[3];   // y face states of C (and D once formed)
+    Flux G[3];         // y flux of the face behind C (and between C and D)
+    float BY[3];       // BY of a row = the bed of its south face
+
+    // prologue (phase 0): rows j-2 .. j+2 (both directions)
     const Row A = finish<Stage, FM>(d, L, fetch<Stage, FM>(d, L, j - 2 * step), j - 2 * step, dt, inv_dt);
     const Raw rB = fetch<Stage, FM>(d, L, j - step);
     const Raw rC = fetch<Stage, FM>(d, L, j);
     const Raw rD = fetch<Stage, FM>(d, L, j + step);
-    Raw rE = fetch<Stage, FM>(d, L, j + 2 * step);
-    Src src = fetch_src<FM>(d, L, j);
+    W[2] = fetch<Stage, FM>(d, L, j + 2 * step);
+    SR[0] = fetch_src<FM>(d, L, j);
     const Row B = finish<Stage, FM>(d, L, rB, j - step, dt, inv_dt);
-    Row C = finish<Stage, FM>(d, L, rC, j, dt, inv_dt);
-    Row D = finish<Stage, FM>(d, L, rD, j + step, dt, inv_dt);
-    FaceStates yC;
-    Flux gB;          // flux of the face between B and C
-    float byC = rC.by, byD = rD.by, byB = rB.by;
-    if constexpr (!Up) {
-        const FaceStates yB = construct<Textbook>(A.h, B.h, C.h, A.b, B.b, C.b, A.hu, B.hu, C.hu,
-                                                  A.hv, B.hv, C.hv, byC, rB.by);
-        yC = construct<Textbook>(B.h, C.h, D.h, B.b, C.b, D.b, B.hu, C.hu, D.hu, B.hv, C.hv, D.hv,
-                                 byD, byC);
-        gB = face_flux<1>(yB.hp, yB.qxp, yB.qyp, yC.hm, yC.qxm, yC.qym, kappa);
-    } else {
-        // rows: A = j+2, B = j+1 (north of C), D = j-1
-        const float byA = ld(d.BY + row_of(d, j + 2), L.c4);
-        const FaceStates yB = construct<Textbook>(C.h, B.h, A.h, C.b, B.b, A.b, C.hu, B.hu, A.hu,
-                                                  C.hv, B.hv, A.hv, byA, byB);
-        yC = construct<Textbook>(D.h, C.h, B.h, D.b, C.b, B.b, D.hu, C.hu, B.hu, D.hv, C.hv, B.hv,
-                                 byB, byC);
-        gB = face_flux<1>(yC.hp, yC.qxp, yC.qyp, yB.hm, yB.qxm, yB.qym, kappa);
+    R[0] = finish<Stage, FM>(d, L, rC, j, dt, inv_dt);
+    R[1] = finish<Stage, FM>(d, L, rD, j + step, dt, inv_dt);
+    BY[0] = rC.by;
+    BY[1] = rD.by;
+    BY[2] = rB.by;
+    {
+        const Row& C = R[0];
+        const Row& D = R[1];
+        if constexpr (!Up) {
+            const FaceStates yB = construct<Textbook>(A.h, B.h, C.h, A.b, B.b, C.b, A.hu, B.hu, C.hu,
+                                                      A.hv, B.hv, C.hv, rC.by, rB.by);
+            Y[0] = construct<Textbook>(B.h, C.h, D.h, B.b, C.b, D.b, B.hu, C.hu, D.hu, B.hv, C.hv, D.hv,
+                                       rD.by, rC.by);
+            G[0] = face_flux<1>(yB.hp, yB.qxp, yB.qyp, Y[0].hm, Y[0].qxm, Y[0].qym, kappa);
+        } else {
+            // rows: A = j+2, B = j+1 (north of C), D = j-1
+            const float byA = ld(d.BY + row_of(d, j + 2), L.c4);
+            const FaceStates yB = construct<Textbook>(C.h, B.h, A.h, C.b, B.b, A.b, C.hu, B.hu, A.hu,
+                                                      C.hv, B.hv, A.hv, byA, rB.by);
+            Y[0] = construct<Textbook>(D.h, C.h, B.h, D.b, C.b, B.b, D.hu, C.hu, B.hu, D.hv, C.hv, B.hv,
+                                       rB.by, rC.by);
+            G[0] = face_flux<1>(Y[0].hp, Y[0].qxp, Y[0].qyp, yB.hm, yB.qxm, yB.qym, kappa);
+        }
     }
 
     float mxl = 0.f, myl = 0.f;
     bool wetl = false;
     BudgetAcc acc{0.f, 0.f, 0.f, 0.f};
-    for (; j != jstop; j += step) {
+    // output row j in phase k; returns whether the run is done
+    auto row = [&](auto phase) __attribute__((always_inline)) -> bool {
+        constexpr int kc = decltype(phase)::value;
+        constexpr int kd = (kc + 1) % 3, ke = (kc + 2) % 3;
         const int rj = (j - 2) / kTileH;   // uniform
         if (rj != rr) {
             flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane);
@@ -560,24 +598,28 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         // (clamped to the domain; unused past the end of the run)
         const int jn = Up ? max(j - 3, 0) : j + 3;
         const Raw rN = fetch<Stage, FM>(d, L, jn);
-        const Src srcN = fetch_src<FM>(d, L, min(max(j + step, 2), d.ny - 3));
+        SR[kd] = fetch_src<FM>(d, L, min(max(j + step, 2), d.ny - 3));
 
-        const Row E = finish<Stage, FM>(d, L, rE, j + 2 * step, dt, inv_dt);
-        const float byE = rE.by;
-        FaceStates yD;
-        Flux gD;      // flux of the face between C and D
+        const float by_behind = BY[ke];    // BY of row j - step
+        R[ke] = finish<Stage, FM>(d, L, W[ke], j + 2 * step, dt, inv_dt);
+        BY[ke] = W[ke].by;
+        W[kc] = rN;
+        const Row& C = R[kc];
+        const Row& D = R[kd];
+        const Row& E = R[ke];
         if constexpr (!Up) {
-            yD = construct<Textbook>(C.h, D.h, E.h, C.b, D.b, E.b, C.hu, D.hu, E.hu, C.hv, D.hv, E.hv,
-                                     byE, byD);
-            gD = face_flux<1>(yC.hp, yC.qxp, yC.qyp, yD.hm, yD.qxm, yD.qym, kappa);
+            Y[kd] = construct<Textbook>(C.h, D.h, E.h, C.b, D.b, E.b, C.hu, D.hu, E.hu, C.hv, D.hv, E.hv,
+                                        BY[ke], BY[kd]);
+            G[kd] = face_flux<1>(Y[kc].hp, Y[kc].qxp, Y[kc].qyp, Y[kd].hm, Y[kd].qxm, Y[kd].qym, kappa);
         } else {
-            yD = construct<Textbook>(E.h, D.h, C.h, E.b, D.b, C.b, E.hu, D.hu, C.hu, E.hv, D.hv, C.hv,
-                                     byC, byD);
-            gD = face_flux<1>(yD.hp, yD.qxp, yD.qyp, yC.hm, yC.qxm, yC.qym, kappa);
+            Y[kd] = construct<Textbook>(E.h, D.h, C.h, E.b, D.b, C.b, E.hu, D.hu, C.hu, E.hv, D.hv, C.hv,
+                                        BY[kc], BY[kd]);
+            G[kd] = face_flux<1>(Y[kd].hp, Y[kd].qxp, Y[kd].qyp, Y[kc].hm, Y[kc].qxm, Y[kc].qym, kappa);
         }
         // north / south faces of row j
-        const Flux& gn = Up ? gB : gD;
-        const Flux& gs = Up ? gD : gB;
+        const Flux& gn = Up ? G[kc] : G[kd];
+        const Flux& gs = Up ? G[kd] : G[kc];
+        const Src& src = SR[kc];
 
         // x pass of row j
         const float bxw = src.bx;
@@ -597,20 +639,36 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             mxl = fmaxf(mxl, fE.speed);
             myl = fmaxf(myl, gn.speed);
             // beds of the south / north faces of row j
-            const float bys = byC, byn = Up ? byB : byD;
+            const float bys = BY[kc], byn = Up ? by_behind : BY[kd];
             wetl |= output_row<Stage, Budget, FM>(d, S, L, c, j, C, src, dh, dhu, dhv, bxw, bxe, bys, byn,
-                                              fE.f0, fw0, gn.f0, gs.f0, acc);
+                                                  fE.f0, fw0, gn.f0, gs.f0, acc);
         }
-        // roll the window
-        if constexpr (Up) byB = byC;
-        C = D;
-        D = E;
-        byC = byD;
-        byD = byE;
-        yC = yD;
-        gB = gD;
-        rE = rN;
-        src = srcN;
+        j += step;
+        return j == jstop;
+    };
+    if (j != jstop) {
+        if constexpr (phased_rows<FM>()) {
+            for (;;) {
+                if (row(Phase<0>{})) break;
+                if (row(Phase<1>{})) break;
+                if (row(Phase<2>{})) break;
+            }
+        } else {
+            // one phase, the window advanced by copies
+            for (;;) {
+                if (row(Phase<0>{})) break;
+                R[0] = R[1];
+                R[1] = R[2];
+                W[2] = W[0];
+                SR[0] = SR[1];
+                Y[0] = Y[1];
+                G[0] = G[1];
+                const float by0 = BY[0];
+                BY[0] = BY[1];
+                BY[1] = BY[2];
+                BY[2] = by0;
+            }
+        }
     }
     flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane);
     if constexpr (Budget) {
@@ -879,8 +937,9 @@ void launch_grow2(const Dev& d, hipStream_t s)
 }
 
 // Launch variants (all give identical results; they differ in register
-// budget and work order): bit 0 = stage 1 at 4 instead of 5 waves per SIMD,
-// bit 1 = stage 2 at 3 instead of 4 waves per SIMD; the dealing of work
+// budget and work order): bit 0 = stage 1 at 5 instead of 4 waves per SIMD,
+// bit 1 = stage 2 at 4 instead of 3 waves per SIMD (the phased row loops
+// need more than 96 and 128 VGPRs, so the fewer waves are the default); the dealing of work
 // shares and their least size are Dev::band and Dev::min_rows.  The grid is
 // the number of workgroups resident at once (march_resident_blocks).
 // The lean instantiation serves the non-textbook domains that use none of
@@ -912,11 +971,11 @@ void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int block
 {
     blocks = std::max(1, blocks);
     if (stage == 1) {
-        if (variant & 1) launch_march_w<1, 4>(d, blocks, s);
-        else launch_march_w<1, 5>(d, blocks, s);
+        if (variant & 1) launch_march_w<1, 5>(d, blocks, s);
+        else launch_march_w<1, 4>(d, blocks, s);
     } else {
-        if (variant & 2) launch_march_w<2, 3>(d, blocks, s);
-        else launch_march_w<2, 4>(d, blocks, s);
+        if (variant & 2) launch_march_w<2, 4>(d, blocks, s);
+        else launch_march_w<2, 3>(d, blocks, s);
     }
 }
 
@@ -941,8 +1000,8 @@ static int resident(unsigned flags, int cus)
 
 int march_resident_blocks(int stage, int variant, unsigned flags, int cus)
 {
-    if (stage == 1) return (variant & 1) ? resident<1, 4>(flags, cus) : resident<1, 5>(flags, cus);
-    return (variant & 2) ? resident<2, 3>(flags, cus) : resident<2, 4>(flags, cus);
+    if (stage == 1) return (variant & 1) ? resident<1, 5>(flags, cus) : resident<1, 4>(flags, cus);
+    return (variant & 2) ? resident<2, 4>(flags, cus) : resident<2, 3>(flags, cus);
 }
 
 void launch_canonicalize(const Dev& d, hipStream_t s)
