@@ -650,6 +650,20 @@ static int step_fused(rift_grid* g)
     return check_launch("fused step");
 }
 
+// Both step graphs captured now (capture records the launches without
+// running them, so the state does not move): a later rift_step then only
+// replays graphs, whatever its step count.
+static int prepare_graphs(rift_grid* g)
+{
+    if (!g->inited || g->prof || !g->use_graphs || g->per_phase || (g->d.flags & RIFT_EULER))
+        return RIFT_OK;
+    prime(g);
+    int rc;
+    if (!g->graphN && (rc = capture_steps(g, kGraphSteps, &g->graphN))) return rc;
+    if (!g->graph1 && (rc = capture_steps(g, 1, &g->graph1))) return rc;
+    return RIFT_OK;
+}
+
 int rift_step(rift_grid* g, int nsteps)
 {
     if (!g) return fail(RIFT_E_ARG, "null handle");
@@ -715,7 +729,7 @@ int rift_set_option(rift_grid* g, int option, int value)
         g->variant = value & (kMarchVariants - 1);
         g->d.band = (value & kMarchVariants) ? 0 : 1;
         g->d.min_rows = (value & (2 * kMarchVariants)) ? kTileH / 2 : kTileH;
-        return RIFT_OK;
+        return prepare_graphs(g);
     default:
         return fail(RIFT_E_ARG, "rift_set_option: unknown option");
     }
@@ -765,8 +779,8 @@ int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
     for (int v = 0; v < RIFT_NVARIANTS && rc == RIFT_OK; ++v) {
         // warm-up from the snapshot (captures the variant's graphs), then
         // the timed run from the same snapshot
-        if ((rc = copy_state(g, c)) || (rc = rift_set_option(c, RIFT_OPT_VARIANT, v))) break;
         c->per_phase = g->per_phase;
+        if ((rc = copy_state(g, c)) || (rc = rift_set_option(c, RIFT_OPT_VARIANT, v))) break;
         if ((rc = rift_step(c, nsteps)) || (rc = copy_state(g, c))) break;
         if ((rc = rift_step(c, 2))) break;
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(RIFT_E_HIP, "hipEventRecord"); break; }
