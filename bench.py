@@ -254,7 +254,7 @@ def main():
         g.set_variant(args.variant)
         variant = args.variant
     else:
-        variant, variant_ms = g.autotune(16)
+        variant, variant_ms = g.autotune(64)
         print(f"[bench] autotuned launch variant {variant}: "
               + " ".join(f"{m:.4f}" for m in variant_ms) + " ms/step", file=sys.stderr, flush=True)
 

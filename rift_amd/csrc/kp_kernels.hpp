@@ -190,9 +190,12 @@ constexpr int kMarchVariants = 4;
 // Ensembles (devs: device array of the members' Dev)
 void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s);
 void launch_timestep2_ens(const Dev* devs, int nmem, int max_tiles, hipStream_t s);
-void launch_march_ens(const Dev* devs, int nmem, unsigned flags, int stage, int min_rows, int blocks,
-                      hipStream_t s);
-int march_ens_resident_blocks(int stage, bool textbook, bool budget, int cus);
+// lean: every member runs the lean instantiation (march_lean of its flags)
+void launch_march_ens(const Dev* devs, int nmem, unsigned flags, bool lean, int stage, int min_rows,
+                      int blocks, hipStream_t s);
+int march_ens_resident_blocks(int stage, bool textbook, bool budget, bool lean, int cus);
+// the lean marching kernels serve these flags
+bool march_lean(unsigned flags);
 void launch_canonicalize(const Dev& d, hipStream_t s);
 
 }  // namespace rift

@@ -209,6 +209,11 @@ __device__ __forceinline__ size_t row_of(const Dev& d, int j)
 constexpr unsigned kLeanMask = ~(RIFT_RAIN_GRID | RIFT_DRAIN_GRID | RIFT_SURGE_GRID | RIFT_HYDROGRAPH |
                                  RIFT_SAVE_MAX | RIFT_SAVE_ARRIVAL);
 constexpr unsigned kFullMask = ~0u;
+// The same features for the ensemble kernels.  Bit 31 is no flag, so the
+// masking is identical; the distinct value keeps the one-body row loop and
+// the scalar-base addressing there (their member lookup leaves no registers
+// for three bodies).
+constexpr unsigned kLeanEnsMask = kLeanMask & ~(1u << 31);
 
 // A lane's column offset as the loads and stores of a row use it.  Left to
 // itself the compiler hoists field + column out of the row loop as a 64-bit
@@ -1113,7 +1118,7 @@ __device__ __forceinline__ void find_member(const Dev* __restrict__ devs, int nm
     base = cbase + __shfl(incl, l) - mcnt;
 }
 
-template <int Stage, bool Textbook, int Waves, bool Budget>
+template <int Stage, bool Textbook, int Waves, bool Budget, bool Lean>
 __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict__ devs, int nmem, int min_rows)
 {
     const int lane = threadIdx.x & 63;
@@ -1170,7 +1175,8 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
         const int rlast = last - s * d.nurows;
         const long long pe = min(pend, (base + kk + 1) * kTileH);
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (base + kk) * kTileH), d.ny - 2);
-        if (ja < jb) march_run<Stage, Textbook, false, Budget, kFullMask>(d, S, s, ja, jb, lane);
+        if (ja < jb)
+            march_run<Stage, Textbook, false, Budget, Lean ? kLeanEnsMask : kFullMask>(d, S, s, ja, jb, lane);
         k = base + kk + 1;
         a = 0;
         if (k * kTileH >= pend) break;
@@ -1189,46 +1195,74 @@ void launch_timestep2_ens(const Dev* devs, int nmem, int max_tiles, hipStream_t 
     hipLaunchKernelGGL(k_timestep2_ens, dim3(blocks, nmem), dim3(256), 0, s, devs);
 }
 
-template <int Stage, int Waves>
+template <int Stage, int Waves, bool Lean>
+static const void* march_ens_kernel(bool textbook, bool budget)
+{
+    if constexpr (Lean)
+        return budget ? (const void*)k_march_ens<Stage, false, Waves, true, true>
+                      : (const void*)k_march_ens<Stage, false, Waves, false, true>;
+    else
+        return textbook ? (budget ? (const void*)k_march_ens<Stage, true, Waves, true, false>
+                                  : (const void*)k_march_ens<Stage, true, Waves, false, false>)
+                        : (budget ? (const void*)k_march_ens<Stage, false, Waves, true, false>
+                                  : (const void*)k_march_ens<Stage, false, Waves, false, false>);
+}
+
+template <int Stage, int Waves, bool Lean>
 static void launch_march_ens_w(const Dev* devs, int nmem, unsigned flags, int min_rows, int blocks,
                                hipStream_t s)
 {
     const bool tb = flags & RIFT_TEXTBOOK, bud = flags & RIFT_MASS_BUDGET;
     const dim3 g(blocks), b(256);
-    if (tb && bud) hipLaunchKernelGGL((k_march_ens<Stage, true, Waves, true>), g, b, 0, s, devs, nmem, min_rows);
-    else if (tb) hipLaunchKernelGGL((k_march_ens<Stage, true, Waves, false>), g, b, 0, s, devs, nmem, min_rows);
-    else if (bud) hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, true>), g, b, 0, s, devs, nmem, min_rows);
-    else hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, false>), g, b, 0, s, devs, nmem, min_rows);
+    if constexpr (Lean) {
+        if (bud) hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, true, true>), g, b, 0, s, devs, nmem, min_rows);
+        else hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, false, true>), g, b, 0, s, devs, nmem, min_rows);
+    } else if (tb && bud) {
+        hipLaunchKernelGGL((k_march_ens<Stage, true, Waves, true, false>), g, b, 0, s, devs, nmem, min_rows);
+    } else if (tb) {
+        hipLaunchKernelGGL((k_march_ens<Stage, true, Waves, false, false>), g, b, 0, s, devs, nmem, min_rows);
+    } else if (bud) {
+        hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, true, false>), g, b, 0, s, devs, nmem, min_rows);
+    } else {
+        hipLaunchKernelGGL((k_march_ens<Stage, false, Waves, false, false>), g, b, 0, s, devs, nmem, min_rows);
+    }
 }
 
 // Ensembles use stage 1 at 5 waves per SIMD and stage 2 at 4 (3 with the
-// volume budget, whose accumulators would spill at 4).
-void launch_march_ens(const Dev* devs, int nmem, unsigned flags, int stage, int min_rows, int blocks,
-                      hipStream_t s)
+// volume budget, whose accumulators would spill at 4).  Lean: every member's
+// flags leave the lean instantiation's features unused.
+void launch_march_ens(const Dev* devs, int nmem, unsigned flags, bool lean, int stage, int min_rows,
+                      int blocks, hipStream_t s)
 {
     blocks = std::max(1, blocks);
-    if (stage == 1) launch_march_ens_w<1, 5>(devs, nmem, flags, min_rows, blocks, s);
-    else if (flags & RIFT_MASS_BUDGET) launch_march_ens_w<2, 3>(devs, nmem, flags, min_rows, blocks, s);
-    else launch_march_ens_w<2, 4>(devs, nmem, flags, min_rows, blocks, s);
+    if (lean) {
+        if (stage == 1) launch_march_ens_w<1, 5, true>(devs, nmem, flags, min_rows, blocks, s);
+        else if (flags & RIFT_MASS_BUDGET) launch_march_ens_w<2, 3, true>(devs, nmem, flags, min_rows, blocks, s);
+        else launch_march_ens_w<2, 4, true>(devs, nmem, flags, min_rows, blocks, s);
+    } else {
+        if (stage == 1) launch_march_ens_w<1, 5, false>(devs, nmem, flags, min_rows, blocks, s);
+        else if (flags & RIFT_MASS_BUDGET) launch_march_ens_w<2, 3, false>(devs, nmem, flags, min_rows, blocks, s);
+        else launch_march_ens_w<2, 4, false>(devs, nmem, flags, min_rows, blocks, s);
+    }
 }
 
-template <int Stage, int Waves>
+template <int Stage, int Waves, bool Lean>
 static int resident_ens(bool textbook, bool budget, int cus)
 {
-    const void* k = textbook ? (budget ? (const void*)k_march_ens<Stage, true, Waves, true>
-                                       : (const void*)k_march_ens<Stage, true, Waves, false>)
-                             : (budget ? (const void*)k_march_ens<Stage, false, Waves, true>
-                                       : (const void*)k_march_ens<Stage, false, Waves, false>);
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
+    hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, march_ens_kernel<Stage, Waves, Lean>(textbook, budget), 256, 0);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     return per_cu * cus;
 }
 
-int march_ens_resident_blocks(int stage, bool textbook, bool budget, int cus)
+int march_ens_resident_blocks(int stage, bool textbook, bool budget, bool lean, int cus)
 {
-    if (stage == 1) return resident_ens<1, 5>(textbook, budget, cus);
-    return budget ? resident_ens<2, 3>(textbook, budget, cus) : resident_ens<2, 4>(textbook, budget, cus);
+    if (stage == 1)
+        return lean ? resident_ens<1, 5, true>(textbook, budget, cus) : resident_ens<1, 5, false>(textbook, budget, cus);
+    if (lean)
+        return budget ? resident_ens<2, 3, true>(textbook, budget, cus) : resident_ens<2, 4, true>(textbook, budget, cus);
+    return budget ? resident_ens<2, 3, false>(textbook, budget, cus) : resident_ens<2, 4, false>(textbook, budget, cus);
 }
 
 }  // namespace rift

@@ -1163,7 +1163,7 @@ struct rift_ensemble {
     Dev* devs = nullptr;            // device copy of the members' Dev
     std::vector<Dev> host_devs;
     unsigned flags = 0;             // TEXTBOOK / MASS_BUDGET (common to all members)
-    int march_blocks[2] = {0, 0};
+    int march_blocks[2][2] = {{0, 0}, {0, 0}};   // [full|lean][stage]
     int max_threads = 0, max_tiles = 0;
     int min_rows = kTileH;
     std::vector<Dev> uploaded;      // what devs holds (the graphs read it)
@@ -1177,10 +1177,15 @@ static void ens_step_launches(rift_ensemble* e)
     const int n = (int)e->members.size();
     for (rift_grid* g : e->members)
         if (g->d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(g->d, e->stream);
+    // the lean kernels when no member uses a feature outside them (a member
+    // whose flags change changes its Dev, which drops the captured graph)
+    bool lean = true;
+    for (rift_grid* g : e->members) lean = lean && march_lean(g->d.flags);
+    const int* blocks = e->march_blocks[lean ? 1 : 0];
     launch_grow2_ens(e->devs, n, e->max_threads, e->stream);
-    launch_march_ens(e->devs, n, e->flags, 1, e->min_rows, e->march_blocks[0], e->stream);
+    launch_march_ens(e->devs, n, e->flags, lean, 1, e->min_rows, blocks[0], e->stream);
     launch_timestep2_ens(e->devs, n, e->max_tiles, e->stream);
-    launch_march_ens(e->devs, n, e->flags, 2, e->min_rows, e->march_blocks[1], e->stream);
+    launch_march_ens(e->devs, n, e->flags, lean, 2, e->min_rows, blocks[1], e->stream);
 }
 
 extern "C" {
@@ -1226,9 +1231,10 @@ int rift_ensemble_create(int n, const rift_desc* descs, rift_ensemble** out)
         e->max_tiles = std::max(e->max_tiles, g->d.ntiles);
     }
     const int cus = e->members[0]->cus;
-    for (int st = 0; st < 2; ++st)
-        e->march_blocks[st] = march_ens_resident_blocks(st + 1, (e->flags & RIFT_TEXTBOOK) != 0,
-                                                        (e->flags & RIFT_MASS_BUDGET) != 0, cus);
+    for (int lean = 0; lean < 2; ++lean)
+        for (int st = 0; st < 2; ++st)
+            e->march_blocks[lean][st] = march_ens_resident_blocks(
+                st + 1, (e->flags & RIFT_TEXTBOOK) != 0, (e->flags & RIFT_MASS_BUDGET) != 0, lean != 0, cus);
     *out = e;
     return RIFT_OK;
 }

@@ -2,7 +2,9 @@
 inputs advanced by one launch per phase give, member by member, exactly the
 results of the same domain stepped alone (bit for bit: the same per-cell
 arithmetic, only the dealing of work to waves differs), each with its own dt;
-end times stop members independently."""
+end times stop members independently.  Rain + Green-Ampt members run the
+lean ensemble kernels; with summary maps (SAVE_MAX) the full ones; with the
+FP64 volume budget the budget instantiations."""
 import numpy as np
 import pytest
 
@@ -26,21 +28,21 @@ def _members():
     return out
 
 
-def _flags(rift):
-    return rift.TILE_ACCEL | rift.RAIN_AVG
+def _flags(rift, extra=0):
+    return rift.TILE_ACCEL | rift.RAIN_AVG | extra
 
 
-def _single(rift, m):
+def _single(rift, m, extra=0):
     g = rift.grid_from_rasters(m["dem"], m["dx"], n_const=m["n_const"], K=m["K"], psi=m["psi"],
-                               dtheta=m["dtheta"], flags=_flags(rift))
+                               dtheta=m["dtheta"], flags=_flags(rift, extra))
     t, v = synth.series_hours_to_seconds(m["hours"], m["mmhr"])
     g.set_series(rift.SER_HYETO, t, v)
     return g
 
 
-def _ensemble(rift, members):
+def _ensemble(rift, members, extra=0):
     spec = [dict(dem=m["dem"], dx=m["dx"], n_const=m["n_const"], K=m["K"], psi=m["psi"],
-                 dtheta=m["dtheta"], flags=_flags(rift)) for m in members]
+                 dtheta=m["dtheta"], flags=_flags(rift, extra)) for m in members]
     e = rift.Ensemble(spec)
     for g, m in zip(e.members, members):
         t, v = synth.series_hours_to_seconds(m["hours"], m["mmhr"])
@@ -48,11 +50,13 @@ def _ensemble(rift, members):
     return e
 
 
-def test_ensemble_members_match_single_domain_runs(rift):
+@pytest.mark.parametrize("extra", ["none", "SAVE_MAX", "MASS_BUDGET"])
+def test_ensemble_members_match_single_domain_runs(rift, extra):
     members = _members()
-    print("ensemble of", len(members), flush=True)
-    e = _ensemble(rift, members)
-    singles = [_single(rift, m) for m in members]
+    print("ensemble of", len(members), extra, flush=True)
+    xf = 0 if extra == "none" else getattr(rift, extra)
+    e = _ensemble(rift, members, xf)
+    singles = [_single(rift, m, xf) for m in members]
     for chunk in (1, 7, 40, 32):
         e.step(chunk)
         for g in singles:
@@ -61,7 +65,7 @@ def test_ensemble_members_match_single_domain_runs(rift):
         for i, (ge, gs) in enumerate(zip(e.members, singles)):
             assert ge.time == gs.time, (i, ge.time, gs.time)
             times.add(ge.time[1])
-            for f in ("depth", "hu", "hv", "F"):
+            for f in ("depth", "hu", "hv", "F") + (("h_max", "q_max") if extra == "SAVE_MAX" else ()):
                 np.testing.assert_array_equal(ge.field(f), gs.field(f), err_msg=f"{i} {f}")
             np.testing.assert_array_equal(ge.tiles()[0], gs.tiles()[0])
             np.testing.assert_array_equal(ge.tiles()[1], gs.tiles()[1])
