@@ -194,7 +194,10 @@ int rift_set_time(rift_grid* g, double t, double dt);
 
 /* Field copies to/from the host (nx*ny doubles; blocking).  Replaces the
  * cudaMemcpy2D transfers of PrintData, PrintSummaryData and the hotstart
- * reader/writer (simulator.cu:568-697, 912-1090). */
+ * reader/writer (simulator.cu:568-697, 912-1090).  Ghost cells of RIFT_W
+ * hold the reference's values: a ghost's surface is the bed of the adjacent
+ * interior cell plus the outflow depth (ApplyBoundaries, grid.cu:1325-1411);
+ * RIFT_DEPTH reports that outflow depth itself there. */
 int rift_download(rift_grid* g, int field, double* host);
 int rift_upload(rift_grid* g, int field, const double* host);
 

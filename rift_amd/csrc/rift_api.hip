@@ -861,6 +861,19 @@ static float* field_plane(rift_grid* g, int field)
     }
 }
 
+// The cell whose bed carries the water surface of cell (i, j).  The outflow
+// boundary stores a ghost cell's depth as the critical depth H over the
+// adjacent interior cell, whose bed the reference adds (ghost w = BC_int + H,
+// grid.cu:1325-1411); corner ghosts are never written and keep their own bed.
+static inline void surface_bed_cell(int nx, int ny, int i, int j, int& bi, int& bj)
+{
+    const bool gx = i < 2 || i > nx - 3, gy = j < 2 || j > ny - 3;
+    bi = i;
+    bj = j;
+    if (gx && !gy) bi = i < 2 ? 2 : nx - 3;
+    if (gy && !gx) bj = j < 2 ? 2 : ny - 3;
+}
+
 int rift_download(rift_grid* g, int field, double* host)
 {
     if (!g || !host) return fail(RIFT_E_ARG, "rift_download: null argument");
@@ -880,8 +893,11 @@ int rift_download(rift_grid* g, int field, double* host)
     if (field == RIFT_W) {
         if ((rc = get_plane(g, d.h, a)) || (rc = get_plane(g, d.BC, b))) return rc;
         for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i)
-                host[(size_t)j * nx + i] = (double)at(a, i, j) + (double)at(b, i, j);
+            for (int i = 0; i < nx; ++i) {
+                int bi, bj;
+                surface_bed_cell(nx, ny, i, j, bi, bj);
+                host[(size_t)j * nx + i] = (double)at(a, i, j) + (double)at(b, bi, bj);
+            }
         return RIFT_OK;
     }
     if (field == RIFT_Q) {
@@ -955,14 +971,18 @@ int rift_upload(rift_grid* g, int field, const double* host)
     Dev& d = g->d;
     const int nx = d.nx, ny = d.ny;
     if (field == RIFT_W) {
-        // depth = w - BC with the device's cell beds
+        // depth = w - BC with the device's cell beds (a ghost's over the
+        // bed of the interior cell its surface is defined on)
         std::vector<float> bc;
         int rc = get_plane(g, d.BC, bc);
         if (rc) return rc;
         std::vector<double> hh((size_t)nx * ny);
         for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i)
-                hh[(size_t)j * nx + i] = host[(size_t)j * nx + i] - (double)bc[(size_t)j * d.P + i];
+            for (int i = 0; i < nx; ++i) {
+                int bi, bj;
+                surface_bed_cell(nx, ny, i, j, bi, bj);
+                hh[(size_t)j * nx + i] = host[(size_t)j * nx + i] - (double)bc[(size_t)bj * d.P + bi];
+            }
         return put_field(g, d.h, hh.data(), nx, ny, nx);
     }
     float* p = field_plane(g, field);
