@@ -961,10 +961,11 @@ void launch_grow2(const Dev& d, hipStream_t s)
 }
 
 // Launch variants (all give identical results; they differ in register
-// budget and work order): bit 0 = stage 1 at 5 instead of 4 waves per SIMD,
-// bit 1 = stage 2 at 4 instead of 3 waves per SIMD (the phased row loops
-// need more than 96 and 128 VGPRs, so the fewer waves are the default); the dealing of work
-// shares and their least size are Dev::band and Dev::min_rows.  The grid is
+// budget and work order): bit 0 = stage 1 launch bounds of 5 instead of 4
+// waves per SIMD, bit 1 = stage 2 at 4 instead of 3 (the lean phased stage
+// 2 needs 131 VGPRs, so 3 is the default; the lean stage 1 fits 87 and runs
+// 5 resident either way); the dealing of work shares and their least size
+// are Dev::band and Dev::min_rows.  The grid is
 // the number of workgroups resident at once (march_resident_blocks).
 // The lean instantiation serves the non-textbook domains that use none of
 // the features outside kLeanMask.
