@@ -1097,91 +1097,131 @@ __device__ __forceinline__ void member_chunk(const Dev* __restrict__ devs, int n
     }
 }
 
-// Locates flat unit index k (k < total): member, base (flat index of the
-// member's first unit) and count.  *chunk / *cbase / cnt / incl cache the
-// 64-member chunk of the last lookup; k never decreases.
-template <int Stage>
-__device__ __forceinline__ void find_member(const Dev* __restrict__ devs, int nmem, int lane, long long k,
-                                            int& c0, long long& cbase, int& cnt, long long& incl,
-                                            int& mem, long long& base, int& mcnt)
+// Shares of one member for bands of hb rows.  A dense member is dealt like
+// a dense domain in k_march: bands of hb rows across all strips (the last
+// one shorter), one strip of one band per share, in band-major order, so
+// that consecutive shares march the same rows of neighbouring strips at the
+// same time and share their halo columns in L2 (dealt strip-major, the same
+// member ran 40 % slower); bands alternate direction.  A sparse member's
+// active units (strip-major) are cut into runs of hb / 14 units.
+__device__ __forceinline__ int member_run_units(int hb)
 {
-    for (;;) {
-        const long long ctot = __shfl(incl, 63);
-        if (k < cbase + ctot) break;
-        cbase += ctot;
-        c0 += 64;
-        member_chunk<Stage>(devs, nmem, c0, lane, cnt, incl);
+    return max(1, hb / kTileH);
+}
+__device__ __forceinline__ int member_shares(const Dev& d, int cnt, int hb)
+{
+    if (cnt == 0) return 0;
+    if (cnt == d.nunits) return ((d.ny - 4 + hb - 1) / hb) * d.nstrips;
+    const int bu = member_run_units(hb);
+    return (cnt + bu - 1) / bu;
+}
+
+// Unit and share counts of members [c0, c0+64), one member per lane, with
+// the inclusive prefix sum of the shares across the wave.
+template <int Stage>
+__device__ __forceinline__ void member_chunk_sh(const Dev* __restrict__ devs, int nmem, int c0, int lane, int B,
+                                                int& cnt, int& sh, long long& incl)
+{
+    const int m = c0 + lane;
+    cnt = (m < nmem) ? member_units<Stage>(devs[m]) : 0;
+    sh = (m < nmem) ? member_shares(devs[m], cnt, B) : 0;
+    incl = sh;
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
     }
-    const unsigned long long above = __ballot(cbase + incl > k);
-    const int l = (int)__builtin_ctzll(above);
-    mem = c0 + l;
-    mcnt = __shfl(cnt, l);
-    base = cbase + __shfl(incl, l) - mcnt;
 }
 
 template <int Stage, bool Textbook, int Waves, bool Budget, bool Lean>
 __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict__ devs, int nmem, int min_rows)
 {
+    constexpr unsigned FM = Lean ? kLeanEnsMask : kFullMask;
     const int lane = threadIdx.x & 63;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // total units over all members, chunk by chunk
-    long long total = 0;
-    int c0 = 0, cnt = 0;
-    long long incl = 0;
+    const int G = (int)gridDim.x;
+    const int nb = min(G, 8), x = (int)blockIdx.x % nb;
+    const long long W = 4LL * G;   // waves
+    long long units = 0;            // active units over all members
     for (int c = 0; c < nmem; c += 64) {
         int cc;
         long long ic;
         member_chunk<Stage>(devs, nmem, c, lane, cc, ic);
-        if (c == 0) {
-            cnt = cc;
-            incl = ic;
-        }
-        total += __shfl(ic, 63);
+        units += __shfl(ic, 63);
     }
-    total *= kTileH;
-    if (total == 0) return;
-    const int G = (int)gridDim.x;
-    const int nb = min(G, 8), x = (int)blockIdx.x % nb, q = G / nb, r = G % nb;
-    const int pos = x * q + min(x, r) + (int)blockIdx.x / nb;
-    const long long W = 4LL * G;
-    const long long wp = 4LL * pos + wid;
-    const long long nshare = max(1LL, min(W, total / max(min_rows, 1)));
-    const long long i0 = (wp * nshare + W - 1) / W, i1 = ((wp + 1) * nshare + W - 1) / W;
-    if (i0 >= i1) return;
-    const long long p0 = total * i0 / nshare, pend = total * i1 / nshare;
-    if (p0 >= pend) return;
-
-    long long k = p0 / kTileH;
-    int a = (int)(p0 - k * kTileH);
-    long long cbase = 0, base = 0;
-    int mem = 0, mcnt = 0;
-    find_member<Stage>(devs, nmem, lane, k, c0, cbase, cnt, incl, mem, base, mcnt);
-    for (;;) {
-        mem = __builtin_amdgcn_readfirstlane(mem);
-        const Dev& d = devs[mem];
-        const bool dense = mcnt == d.nunits;
-        const StepScalars S = step_scalars(d);
-        // runs of vertically adjacent units of one strip within this member
-        const int kl = (int)(k - base);
-        const int u = __builtin_amdgcn_readfirstlane(dense ? kl : d.ulist[kl]);
-        const int s = u / d.nurows, rr = u - s * d.nurows;
-        const int ja = rr * kTileH + 2 + a;
-        int last = u, kk = kl;
-        while (kk + 1 < mcnt && (base + kk + 1) * kTileH < pend) {
-            const int un = __builtin_amdgcn_readfirstlane(dense ? kk + 1 : d.ulist[kk + 1]);
-            if (un != last + 1 || un - s * d.nurows >= d.nurows) break;
-            last = un;
-            ++kk;
+    if (units == 0) return;
+    // band height in rows: the total rows over the waves, raised in steps
+    // of 1/16 while the shares (rounded up per member) outnumber the waves,
+    // so that about one band of one strip falls to each wave
+    int B = (int)max((long long)max(min_rows, 1), (units * kTileH + W - 1) / W);
+    long long total = 0;
+    for (int attempt = 0;; ++attempt) {
+        total = 0;
+        for (int c = 0; c < nmem; c += 64) {
+            int cc, sc;
+            long long ic;
+            member_chunk_sh<Stage>(devs, nmem, c, lane, B, cc, sc, ic);
+            total += __shfl(ic, 63);
         }
-        const int rlast = last - s * d.nurows;
-        const long long pe = min(pend, (base + kk + 1) * kTileH);
-        const int jb = min(rlast * kTileH + 2 + (int)(pe - (base + kk) * kTileH), d.ny - 2);
-        if (ja < jb)
-            march_run<Stage, Textbook, false, Budget, Lean ? kLeanEnsMask : kFullMask>(d, S, s, ja, jb, lane);
-        k = base + kk + 1;
-        a = 0;
-        if (k * kTileH >= pend) break;
-        if (k >= base + mcnt) find_member<Stage>(devs, nmem, lane, k, c0, cbase, cnt, incl, mem, base, mcnt);
+        if (total <= W || attempt == 8) break;
+        B += max(1, B / 16);
+    }
+    // static dealing in band-major order: wave i takes shares [i T / W, (i + 1) T / W)
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int q = G / nb, r = G % nb;
+    const int pos = x * q + min(x, r) + (int)blockIdx.x / nb;
+    const long long wp = 4LL * pos + wid;
+    const long long i0 = wp * total / W, i1 = (wp + 1) * total / W;
+    if (i0 >= i1) return;
+
+    int c0 = 0, cnt = 0, sh = 0;
+    long long cbase = 0, incl = 0;
+    member_chunk_sh<Stage>(devs, nmem, 0, lane, B, cnt, sh, incl);
+    int smem = -1;
+    StepScalars S{};
+    for (long long i = i0; i < i1; ++i) {
+        for (;;) {   // the 64-member chunk holding share i (i never decreases)
+            const long long ctot = __shfl(incl, 63);
+            if (i < cbase + ctot) break;
+            cbase += ctot;
+            c0 += 64;
+            member_chunk_sh<Stage>(devs, nmem, c0, lane, B, cnt, sh, incl);
+        }
+        const int l = (int)__builtin_ctzll(__ballot(cbase + incl > i));
+        const int mem = __builtin_amdgcn_readfirstlane(c0 + l);
+        const int mcnt = __builtin_amdgcn_readfirstlane(__shfl(cnt, l));
+        const int ls = (int)(i - (cbase + __shfl(incl, l) - __shfl(sh, l)));
+        const Dev& d = devs[mem];
+        if (mem != smem) {
+            S = step_scalars(d);
+            smem = mem;
+        }
+        if (mcnt == d.nunits) {
+            const int band = ls / d.nstrips, s = ls - band * d.nstrips;
+            const int ja = 2 + band * B, jb = min(ja + B, d.ny - 2);
+            if (ja >= jb) continue;
+            if (band & 1) march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane);
+            else march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
+        } else {
+            // units [ls bu, (ls + 1) bu) of the list: runs of vertically
+            // adjacent units of one strip
+            const int bu = member_run_units(B);
+            int kk = ls * bu;
+            const int kend = min(kk + bu, mcnt);
+            while (kk < kend) {
+                const int u = __builtin_amdgcn_readfirstlane(d.ulist[kk]);
+                const int s = u / d.nurows, rr = u - s * d.nurows;
+                int last = u;
+                while (kk + 1 < kend) {
+                    const int un = __builtin_amdgcn_readfirstlane(d.ulist[kk + 1]);
+                    if (un != last + 1 || un - s * d.nurows >= d.nurows) break;
+                    last = un;
+                    ++kk;
+                }
+                const int ja = rr * kTileH + 2;
+                const int jb = min((last - s * d.nurows + 1) * kTileH + 2, d.ny - 2);
+                if (ja < jb) march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
+                ++kk;
+            }
+        }
     }
 }
 
@@ -1229,21 +1269,20 @@ static void launch_march_ens_w(const Dev* devs, int nmem, unsigned flags, int mi
     }
 }
 
-// Ensembles use stage 1 at 5 waves per SIMD and stage 2 at 4 (3 with the
-// volume budget, whose accumulators would spill at 4).  Lean: every member's
-// flags leave the lean instantiation's features unused.
+// Ensembles use stage 1 at 4 waves per SIMD and stage 2 at 3: the kernels
+// hold both marching directions and the member lookup, which spill at 5
+// and 4.  Lean: every member's flags leave the lean instantiation's
+// features unused.
 void launch_march_ens(const Dev* devs, int nmem, unsigned flags, bool lean, int stage, int min_rows,
                       int blocks, hipStream_t s)
 {
     blocks = std::max(1, blocks);
     if (lean) {
-        if (stage == 1) launch_march_ens_w<1, 5, true>(devs, nmem, flags, min_rows, blocks, s);
-        else if (flags & RIFT_MASS_BUDGET) launch_march_ens_w<2, 3, true>(devs, nmem, flags, min_rows, blocks, s);
-        else launch_march_ens_w<2, 4, true>(devs, nmem, flags, min_rows, blocks, s);
+        if (stage == 1) launch_march_ens_w<1, 4, true>(devs, nmem, flags, min_rows, blocks, s);
+        else launch_march_ens_w<2, 3, true>(devs, nmem, flags, min_rows, blocks, s);
     } else {
-        if (stage == 1) launch_march_ens_w<1, 5, false>(devs, nmem, flags, min_rows, blocks, s);
-        else if (flags & RIFT_MASS_BUDGET) launch_march_ens_w<2, 3, false>(devs, nmem, flags, min_rows, blocks, s);
-        else launch_march_ens_w<2, 4, false>(devs, nmem, flags, min_rows, blocks, s);
+        if (stage == 1) launch_march_ens_w<1, 4, false>(devs, nmem, flags, min_rows, blocks, s);
+        else launch_march_ens_w<2, 3, false>(devs, nmem, flags, min_rows, blocks, s);
     }
 }
 
@@ -1260,10 +1299,8 @@ static int resident_ens(bool textbook, bool budget, int cus)
 int march_ens_resident_blocks(int stage, bool textbook, bool budget, bool lean, int cus)
 {
     if (stage == 1)
-        return lean ? resident_ens<1, 5, true>(textbook, budget, cus) : resident_ens<1, 5, false>(textbook, budget, cus);
-    if (lean)
-        return budget ? resident_ens<2, 3, true>(textbook, budget, cus) : resident_ens<2, 4, true>(textbook, budget, cus);
-    return budget ? resident_ens<2, 3, false>(textbook, budget, cus) : resident_ens<2, 4, false>(textbook, budget, cus);
+        return lean ? resident_ens<1, 4, true>(textbook, budget, cus) : resident_ens<1, 4, false>(textbook, budget, cus);
+    return lean ? resident_ens<2, 3, true>(textbook, budget, cus) : resident_ens<2, 3, false>(textbook, budget, cus);
 }
 
 }  // namespace rift
