@@ -1042,9 +1042,10 @@ void launch_canonicalize(const Dev& d, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Ensembles: many domains (scenarios) advanced together, one launch per
 // phase.  devs[] holds every member's Dev in device memory; grow and dt run
-// one grid row per member (blockIdx.y); the marching kernels deal equal row
-// shares of the members' active units, in member order, to the resident
-// waves, so small and large members share the device without a launch each.
+// one grid row per member (blockIdx.y); the marching kernels deal the
+// members' active rows, in member order, to the resident waves (bands across
+// the strips of each fully active member, runs of units of the others), so
+// small and large members share the device without a launch each.
 // Every member keeps its own time, dt, forcing and tile sets; a member's
 // cells see exactly the arithmetic of a single-domain step.
 // ---------------------------------------------------------------------------
