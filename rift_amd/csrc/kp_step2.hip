@@ -1151,8 +1151,13 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
     if (units == 0) return;
     // band height in rows: the total rows over the waves, raised in steps
     // of 1/16 while the shares (rounded up per member) outnumber the waves,
-    // so that about one band of one strip falls to each wave
-    int B = (int)max((long long)max(min_rows, 1), (units * kTileH + W - 1) / W);
+    // so that about one band of one strip falls to each wave.  Where that
+    // does not fit (members a little taller than one band each), a quarter
+    // of it: each wave then takes 4 or 5 shares, instead of some waves two
+    // whole ones and the rest one.
+    const int lo_rows = max(min_rows, 1);
+    const long long rows_all = units * kTileH;
+    int B = (int)max((long long)lo_rows, (rows_all + W - 1) / W);
     long long total = 0;
     for (int attempt = 0;; ++attempt) {
         total = 0;
@@ -1162,7 +1167,18 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
             member_chunk_sh<Stage>(devs, nmem, c, lane, B, cc, sc, ic);
             total += __shfl(ic, 63);
         }
-        if (total <= W || attempt == 8) break;
+        if (total <= W) break;
+        if (attempt == 8) {
+            B = (int)max((long long)lo_rows, (rows_all + 4 * W - 1) / (4 * W));
+            total = 0;
+            for (int c = 0; c < nmem; c += 64) {
+                int cc, sc;
+                long long ic;
+                member_chunk_sh<Stage>(devs, nmem, c, lane, B, cc, sc, ic);
+                total += __shfl(ic, 63);
+            }
+            break;
+        }
         B += max(1, B / 16);
     }
     // static dealing in band-major order: wave i takes shares [i T / W, (i + 1) T / W)
