@@ -93,3 +93,32 @@ def test_ensemble_end_times(rift):
         assert g.time[0] == t
         np.testing.assert_array_equal(g.field("depth"), h)
     e.close()
+
+
+def test_ensemble_of_more_than_64_members(rift):
+    """Members are located 64 at a time (one per lane); 70 small members of
+    varied sizes, some dense and some with few active tiles, still match
+    their single-domain runs bit for bit."""
+    members = []
+    for k in range(70):
+        n, m = 20 + (k * 7) % 37, 18 + (k * 11) % 29
+        dem = 15.0 * synth.fractal_dem(max(n, m), 0.6, 300 + k)[: m + 1, : n + 1]
+        K = 4.0 * np.exp(0.5 * synth.normal(400 + k, (m + 1) * (n + 1)).reshape(m + 1, n + 1))
+        members.append({"dem": dem, "dx": 5.0, "n_const": 0.03 + 0.0005 * k, "K": K, "psi": 0.11,
+                        "dtheta": 0.3, "hours": np.array([0.0, 0.01, 0.03], np.float32),
+                        "mmhr": np.array([0.0, 40.0 + k, 0.0], np.float32)})
+    print("ensemble of", len(members), flush=True)
+    e = _ensemble(rift, members)
+    singles = [_single(rift, m) for m in members]
+    for chunk in (3, 29):
+        e.step(chunk)
+        for g in singles:
+            g.step(chunk)
+        for i, (ge, gs) in enumerate(zip(e.members, singles)):
+            assert ge.time == gs.time, (i, ge.time, gs.time)
+            for f in ("depth", "hu", "hv", "F"):
+                np.testing.assert_array_equal(ge.field(f), gs.field(f), err_msg=f"{i} {f}")
+    for g in singles:
+        g.close()
+    e.close()
+
