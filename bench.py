@@ -227,7 +227,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the solver runs one domain per GPU; see DESIGN.md for how it would shard)",
+              "(the solver runs one domain per GPU)",
               file=sys.stderr)
         sys.exit(2)
 

@@ -386,7 +386,7 @@ INFER_BATCH = 64
 def screen(net, scs, X, heldout_frac=0.05, seed=1, chunk=256, log=None):
     """Surrogate screening of the scenarios (features X): predict every
     scenario, send to the solver those whose predicted asset depth lies
-    within MARGIN of THRESHOLD plus a seeded held-out share, and judge the
+    within MARGIN of THRESHOLD plus a seeded held-out share, and measure the
     surrogate's error on the held-out scenarios.  Returns the decisions,
     the final maps (solver maps where the solver ran) and the timings."""
     pred, t_inf = predict(net, X, min(net.max_batch, INFER_BATCH))
