@@ -204,6 +204,42 @@ def calibration_copy(nbytes=256 << 20):
     return nbytes
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """What a plain device-to-device copy achieves on this GPU: `reps`
+    hipMemcpyAsync of nbytes between two fresh allocations on the null
+    stream, timed with HIP events on that stream after two untimed copies;
+    bytes moved = 2 x nbytes per copy (read + write).  It puts the marching
+    kernels' measured HBM traffic beside an attainable rate, not the
+    published peak alone."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(nbytes)
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0
+    assert hip.hipMalloc(ctypes.byref(b), n) == 0
+    try:
+        assert hip.hipMemset(a, 1, n) == 0
+        assert hip.hipEventCreate(ctypes.byref(e0)) == 0
+        assert hip.hipEventCreate(ctypes.byref(e1)) == 0
+        for _ in range(2):
+            assert hip.hipMemcpyAsync(b, a, n, 3, None) == 0      # hipMemcpyDeviceToDevice
+        assert hip.hipEventRecord(e0, None) == 0
+        for _ in range(reps):
+            assert hip.hipMemcpyAsync(b, a, n, 3, None) == 0
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return {"GBps": 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9, "bytes_per_copy": nbytes,
+            "copies": reps, "ms_per_copy": ms.value / reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,6 +331,8 @@ def main():
                       "achieved_GBps": nbytes / avg_s / 1e9 if avg_s > 0 else 0.0}
     dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
     achieved = kern[dom]["achieved_GBps"] * 1e9
+    traffic = measured_traffic("k_march<2>" if dom == "march2" else "k_march<1>", args.cells)
+    copy_bw = copy_bandwidth()
     line = {
         "metric": "wet-cell updates/s",
         "value": value,
@@ -314,12 +352,14 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic("k_march<2>" if dom == "march2" else "k_march<1>",
-                                                 args.cells),
+                     "traffic": traffic,
                      "traffic_source": os.path.relpath(PROFILE, ROOT),
                      "kernel": "k_march<2>" if dom == "march2" else "k_march<1>",
                      "bytes_per_launch": kern[dom]["bytes_per_launch"],
-                     "avg_launch_ms": kern[dom]["avg_launch_ms"]},
+                     "avg_launch_ms": kern[dom]["avg_launch_ms"],
+                     "traffic_GBps": None if traffic is None
+                     else traffic / (kern[dom]["avg_launch_ms"] * 1e-3) / 1e9},
+        "hbm_copy": copy_bw,
         "kernels": kern,
         "step_roofline": {"bytes_per_wcu": STEP_BYTES_PER_WCU,
                           "achieved_GBps": value * STEP_BYTES_PER_WCU / 1e9,
