@@ -1,26 +1,43 @@
 #!/usr/bin/env python3
 """Benchmark: wet-cell updates per second of the KP step on one MI355X.
 
-Workload (N=1): rain_basin 2048 x 2048 cells, dx = 10 m, diamond-square DEM
-0-60 m with a carved channel, n = 0.035, triangular hyetograph 0 -> 80 mm/hr
-over 1 h, Green-Ampt K lognormal (median 5 mm/hr, sigma_ln 0.8), psi 0.11,
-dtheta 0.3, tile acceleration on, inputs generated with fixed seeds and
-resident in HBM before the timed region.
+Headline (N=1): coastal surge 8192 x 8192 cells, dx = 5 m, ramp -5 -> +30 m
+plus fractal noise (amplitude 4 m), Manning map 0.02 below 0 m / 0.06 above,
+37 surge maps every 600 s on the low edge (2.5 sin(pi t / 2 h), clipped at
+0) held in device slots, hydrograph 0 -> 500 cfs on a 21 x 21 cell
+rectangle, tile acceleration on, tf = 6 h.  The state is spun up to a fixed
+simulated time (t = 3 h, 5-25 % wet) outside the timed region and
+independent of --warmup; the FP64 volume budget runs over the spin-up and
+its residual is reported.  Then W untimed and K timed steps, inputs resident
+in HBM, stream synchronised on both sides.
 
-A step is one full Heun time step (both stages, forcing, tile bookkeeping,
-dt).  A wet-cell update (WCU) is one interior cell of an active 16 x 14 tile
-advanced through one step.  value = WCU over the K timed steps / wall time of
-those steps, bracketed by stream synchronisations.
+A step is one full Heun time step (both stages, forcing, surge window, tile
+bookkeeping, dt).  A wet-cell update (WCU) is one interior cell of an active
+16 x 14 tile advanced through one step.  value = WCU of the K timed steps /
+wall time of those steps.
 
-roofline: the dominant kernel of the fused step (the stage-2 march) timed with
-HIP events on the handle's stream; bytes per launch = wet cells of the launch
-x the per-cell bytes of its contract (KERNEL_BYTES_PER_CELL), against 8 TB/s.
-cpu_baseline: the FP32 OpenMP transcription of the same per-cell update
-(oracle/liboracle32.so) on the same workload, a bounded number of steps.
+roofline: the dominant kernel of the step (the stage-2 march), timed with HIP
+events on the handle's stream in a window right after the timed one; its
+algorithmic bytes per launch = wet cells x the stage-2 bytes per WCU of the
+workload (STAGE_BYTES: every element once per stage, one corner-DEM bed
+field), against 8 TB/s.  step_roofline: the whole step at the workload's
+bytes per WCU.  traffic: HBM bytes per launch of that kernel measured with
+rocprofv3 FETCH_SIZE / WRITE_SIZE on this same command, read from the
+committed profile summary when the run reaches the same state.
+
+Second legs: rain basin 2048 x 2048 at full activity, the batched ensemble
+(64 scenarios), surrogate screening; cpu_baseline: the FP32 OpenMP
+transcription of the same per-cell update (oracle/liboracle32.so) on a
+bounded window of the headline workload.
+
+When run under `rocprofv3 --selected-regions`, only the headline's timed and
+event windows (and the --calibrate copy) are profiled.
 """
 import argparse
+import ctypes
 import json
 import os
+import platform
 import sys
 import time
 
@@ -30,35 +47,121 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
-# Bytes per wet cell that each kernel of the fused step must move by its
-# contract (rain_basin flags: rain, Green-Ampt, uniform Manning n):
-#   march<1>: reads h, hu, hv, BC, BY, BX, K, F (32 B), writes dh, dhu, dhv (12 B)
-#   march<2>: reads h, hu, hv, dh, dhu, dhv, BC, BY, BX, K, F (44 B),
-#             writes h, hu, hv, F (16 B)
-KERNEL_BYTES_PER_CELL = {"march1": 44, "march2": 60}
-# algorithmic bytes per WCU of a whole step for this workload (state once per
-# stage, one bed field, Green-Ampt F and K; uniform n costs nothing)
-STEP_BYTES_PER_WCU = 96
+# Algorithmic bytes per WCU (FP32, each element touched once per stage, one
+# corner-DEM bed field, U^n kept for stage 2):
+#   stage 1: read U^n 12 + write 12 + bed 4 (+ Manning map 4, surge map 4;
+#            rain basin: K 4 + F read/write 8)
+#   stage 2: read U* 12 + U^n 12 + write 12 + bed 4 (+ Manning 4;
+#            rain basin: K 4 + F 12)
+STAGE_BYTES = {
+    "coastal": {"march1": 36, "march2": 44, "step": 80},
+    "rain_basin": {"march1": 40, "march2": 56, "step": 96},
+}
+SPINUP_S = 3 * 3600.0
+COASTAL_PROFILE = os.path.join(ROOT, "profiles", "fused_step_coastal_8192.md")
+RAIN_PROFILE = os.path.join(ROOT, "profiles", "fused_step_rain_basin_2048.md")
 
 
-PROFILE = os.path.join(ROOT, "profiles", "fused_step_rain_basin_2048.md")
+def _json_default(o):
+    """numpy scalars and arrays in the JSON line."""
+    if isinstance(o, np.ndarray):
+        return o.tolist()
+    if isinstance(o, np.generic):
+        return o.item()
+    raise TypeError(f"not JSON serialisable: {type(o).__name__}")
 
 
-def measured_traffic(kernel, n_cells):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
-    WRITE_SIZE (calibrated against a copy of known size) for the 2048 x 2048
-    workload, as recorded in the committed profile summary; None for other
-    workloads or when the summary is absent."""
-    if n_cells != 2048 or not os.path.exists(PROFILE):
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+class Regions:
+    """roctxProfilerPause/Resume: profiled regions under rocprofv3
+    --selected-regions (no-ops when the library is absent)."""
+
+    def __init__(self):
+        self.lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is not None:
+            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
+            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
+
+    def pause(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerPause(0)
+
+    def resume(self):
+        if self.lib is not None:
+            self.lib.roctxProfilerResume(0)
+
+
+class NoRegions:
+    def pause(self):
+        pass
+
+    def resume(self):
+        pass
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def measured_traffic(path, kernel, state):
+    """HBM bytes per launch of `kernel` (rocprofv3 FETCH_SIZE / WRITE_SIZE of
+    this command, calibrated against a copy of known size) from a committed
+    profile summary, when the summary was taken on the same state (same
+    `state` key: workload, simulated time and active tiles); else None."""
+    if not os.path.exists(path):
+        return None
+    want = f"state: {state}"
+    lines = [ln.rstrip("\n") for ln in open(path)]
+    if want not in lines:
         return None
     key = f"traffic_bytes_per_launch {kernel}:"
-    for line in open(PROFILE):
-        if line.startswith(key):
-            return float(line[len(key):].strip())
+    for ln in lines:
+        if ln.startswith(key):
+            return float(ln[len(key):].strip())
     return None
 
 
-def build_workload(n_cells):
+def build_coastal(n_cells, budget=False):
+    """The coastal-surge workload: ramp -5 -> +30 m plus fractal noise, n map
+    0.02 / 0.06, 37 surge maps every 600 s held in device slots (generated
+    one index at a time, never all on the host), hydrograph 0 -> 500 cfs on
+    a 21 x 21 cell rectangle, tile acceleration on.  The series window
+    starts at t = 0."""
+    from rift_amd import kp, synth
+    t0 = time.perf_counter()
+    cfg = synth.coastal_surge(n_cells, with_maps=False)
+    flags = kp.TILE_ACCEL | kp.SURGE_GRID | kp.HYDROGRAPH | (kp.MASS_BUDGET if budget else 0)
+    g = kp.grid_from_rasters(cfg["dem"], cfg["dx"], n=cfg["n"], flags=flags)
+    t, v = synth.series_hours_to_seconds(cfg["hydro_hours"], cfg["hydro_cfs"])
+    g.set_series(kp.SER_HYDRO, t, v)
+    g.set_source_cells(synth.source_cells(g.nx, cfg["source_rect"]))
+    nmaps = int(cfg["surge_tf"] // cfg["surge_dt"]) + 1
+    g.map_series(kp.MAP_SURGE, nmaps, cfg["surge_dt"], cfg["surge_tf"])
+    for k in range(nmaps):
+        g.load_map(kp.MAP_SURGE, k, kp.map_cells(synth.surge_map(n_cells, k, cfg["surge_dt"])))
+    g.start_maps(kp.MAP_SURGE, 0.0)
+    cfg["nmaps"] = nmaps
+    log(f"coastal {n_cells}x{n_cells} ready: {g.ntiles} tiles, {nmaps} surge maps on the device "
+        f"({time.perf_counter() - t0:.0f} s)")
+    return g, cfg
+
+
+def build_rain_basin(n_cells):
     from rift_amd import kp, synth
     cfg = synth.rain_basin(n_cells, seed=2)
     flags = kp.TILE_ACCEL | kp.RAIN_AVG
@@ -69,40 +172,131 @@ def build_workload(n_cells):
     return g, cfg
 
 
-def cpu_baseline(cfg, g, budget_s=15.0, max_steps=60):
-    """Time the FP32 OpenMP transcription on the host cores, continuing from
-    the GPU's state (h, hu, hv, F, tile flags, t, dt copied over like a
-    hotstart), so both legs time the same step window of the same workload."""
+def spin_up(g, t_target, chunk=4096, label="spin-up"):
+    """Steps until the simulated time reaches t_target (whole chunks of
+    graph-replayed steps, then single steps); returns the step count."""
+    steps = 0
+    t = g.time[0]
+    w0 = time.perf_counter()
+    while t < t_target:
+        _, dt = g.time
+        n = max(1, min(chunk, int(0.5 * (t_target - t) / max(dt, 1e-9))))
+        g.step(n)
+        steps += n
+        t = g.time[0]
+        if n == chunk:
+            _, _, nact = g.tiles()
+            log(f"{label}: t = {t:.0f} s, {steps} steps, {nact} active tiles "
+                f"({time.perf_counter() - w0:.0f} s)")
+    return steps
+
+
+def timed_windows(g, steps, warmup, variant, regions, stage_bytes):
+    """W untimed steps, then K timed steps (wall clock between stream
+    synchronisations), then a window of min(K, 100) steps with every kernel
+    bracketed by HIP events on the handle's stream."""
+    if variant >= 0:
+        g.set_variant(variant)
+        variant_ms = None
+    else:
+        variant, variant_ms = g.autotune(64)
+        log("autotuned launch variant %d: %s ms/step" % (variant, " ".join(f"{m:.4f}" for m in variant_ms)))
+    g.step(warmup)
+    g.sync()
+    t_start, _ = g.time
+    _, wcu0 = g.wet_cells()
+    regions.resume()
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    _, wcu1 = g.wet_cells()
+    kp_steps = min(steps, 100)
+    g.profile(True)
+    _, wk0 = g.wet_cells()
+    g.step(kp_steps)
+    g.sync()
+    regions.pause()
+    _, wk1 = g.wet_cells()
+    stats = {c: g.kernel_stats(c) for c in ("flux1", "stage2_flux", "grow", "timestep")}
+    g.profile(False)
+    wet, act, n_act = g.tiles()
+    wcu = wcu1 - wcu0
+    value = wcu / elapsed
+    cells_per_launch = (wk1 - wk0) / kp_steps
+    kern = {}
+    for name, cls in (("march1", "flux1"), ("march2", "stage2_flux")):
+        n, ms = stats[cls]
+        avg_s = ms / max(n, 1) * 1e-3
+        nbytes = cells_per_launch * stage_bytes[name]
+        kern[name] = {"avg_launch_ms": avg_s * 1e3, "launches": n, "bytes_per_launch": nbytes,
+                      "achieved_GBps": nbytes / avg_s / 1e9 if avg_s > 0 else 0.0}
+    dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
+    return {
+        "value": value, "elapsed": elapsed, "ms_per_step": 1e3 * elapsed / max(steps, 1),
+        "wcu": wcu, "t_start": t_start, "n_act": int(n_act), "kern": kern, "dom": dom,
+        "cells_per_launch": cells_per_launch,
+        "kernel_ms_per_step": {"march1": stats["flux1"][1] / kp_steps,
+                               "march2": stats["stage2_flux"][1] / kp_steps,
+                               "grow_track": stats["grow"][1] / kp_steps,
+                               "timestep": stats["timestep"][1] / kp_steps},
+        "variant": variant, "variant_ms": variant_ms,
+    }
+
+
+def roofline(w, stage_bytes, traffic):
+    k = w["kern"][w["dom"]]
+    achieved = k["achieved_GBps"] * 1e9
+    name = "k_march<2>" if w["dom"] == "march2" else "k_march<1>"
+    return {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": name,
+            "bytes_per_wcu": stage_bytes[w["dom"]], "bytes_per_launch": k["bytes_per_launch"],
+            "avg_launch_ms": k["avg_launch_ms"],
+            "traffic_over_algorithmic": None if traffic is None else traffic / k["bytes_per_launch"]}
+
+
+def coastal_cpu_baseline(cfg, g, budget_s=20.0, max_steps=60):
+    """The FP32 OpenMP transcription (oracle/liboracle32.so) continuing from
+    the GPU's state (w, hu, hv, tile flags, t, dt, as a hotstart; the surge
+    series restarts its window at t with the maps around it) on the host
+    cores: a bounded window of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    from rift_amd import kp, synth
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = O.Oracle(cfg["dem"], cfg["dx"], n_const=cfg["n_const"], K=cfg["K"], psi=cfg["psi"],
-                 dtheta=cfg["dtheta"], flags=O.TILE_ACCEL | O.RAIN_AVG, fp32=True)
-    o.set_series_hours(O.SER_HYETO, cfg["hyeto_hours"], cfg["hyeto_mmhr"])
-    bc = o.field("bc")
-    o.set_field("w", g.field("depth") + bc)
-    o.set_field("hu", g.field("hu"))
-    o.set_field("hv", g.field("hv"))
-    o.set_field("F", g.field("F"))
+    n_cells = cfg["dem"].shape[0] - 1
+    t_setup = time.perf_counter()
+    o = O.Oracle(cfg["dem"], cfg["dx"], n=cfg["n"],
+                 flags=O.TILE_ACCEL | O.HYDROGRAPH | O.SURGE_GRID, fp32=True)
+    o.set_sources(synth.source_cells(o.nx, cfg["source_rect"]))
+    o.set_series_hours(O.SER_HYDRO, cfg["hydro_hours"], cfg["hydro_cfs"])
+    for name in ("w", "hu", "hv"):
+        o.set_field(name, g.field(name))
     wet, act, _ = g.tiles()
     o.set_tiles(wet, np.where(act > 0, np.arange(g.ntiles), -np.arange(g.ntiles)))
     t, dt = g.time
     o.set_time(t, dt)
+    k0 = g.map_state(kp.MAP_SURGE)[0]
+    maps = np.stack([synth.surge_map(n_cells, k, cfg["surge_dt"]) for k in range(k0, k0 + 3)])
+    o.set_map_series(O.MAP_SURGE, maps, cfg["surge_dt"], cfg["surge_tf"], first=k0)
+    del maps
     o.step(1)                       # warm-up (also restores the forcing state)
+    setup = time.perf_counter() - t_setup
     cells = 0
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps and time.perf_counter() - t0 < budget_s:
         o.step()
-        wet, act, n_act = o.tiles()
-        cells += _active_interior_cells(o.nx, o.ny, o.nbx, act)
+        _, oact, _ = o.tiles()
+        cells += _active_interior_cells(o.nx, o.ny, o.nbx, oact)
         steps += 1
     el = time.perf_counter() - t0
-    n = cfg["dem"].shape[0] - 1
     return {"value": cells / el, "unit": "WCU/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of rain_basin {n}x{n} continuing from the GPU state after the "
-                      f"GPU's timed window (1 untimed step), FP32 OpenMP transcription of the same "
-                      f"per-cell update (oracle/liboracle32.so), {el:.1f} s, {threads} threads"}
+            "cpu_model": cpu_model(),
+            "sample": f"{steps} steps of coastal {n_cells}x{n_cells} continuing from the GPU state at "
+                      f"t = {t:.0f} s (after the GPU's timed windows; 1 untimed step), FP32 OpenMP "
+                      f"transcription of the same per-cell update and tile logic "
+                      f"(oracle/liboracle32.so), {el:.1f} s, {threads} threads ({setup:.0f} s set-up)"}
 
 
 def ensemble_leg(nscen, seed=4, check_every=64):
@@ -240,21 +434,91 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
             "copies": reps, "ms_per_copy": ms.value / reps}
 
 
+def coastal_leg(args, regions):
+    from rift_amd import kp
+    g, cfg = build_coastal(args.cells, budget=True)
+    spin_steps = spin_up(g, SPINUP_S, label="coastal spin-up")
+    b = g.budget()
+    g.set_budget(False)
+    t_spin, _ = g.time
+    h = g.interior(g.field("depth"))
+    wet_frac = float((h > 1.19209e-07).mean())
+    _, _, n_act0 = g.tiles()
+    log(f"coastal spun up: t = {t_spin:.1f} s after {spin_steps} steps, {n_act0}/{g.ntiles} tiles active, "
+        f"{wet_frac:.3f} of cells wet")
+    log("budget m^3: stored %.6e infiltrated %.3e at reset %.6e rain %.3e source %.6e drain %.3e "
+        "outflow %.6e exchange %.6e residual %.3e" % tuple(b))
+    sb = STAGE_BYTES["coastal"]
+    w = timed_windows(g, args.steps, args.warmup, args.variant, regions, sb)
+    state = f"coastal {args.cells} t0={SPINUP_S:.0f} spin={spin_steps} active={n_act0}"
+    traffic = measured_traffic(COASTAL_PROFILE, "k_march<2>" if w["dom"] == "march2" else "k_march<1>",
+                               state)
+    k0, k1, f, miss = g.map_state(kp.MAP_SURGE)
+    scale = max(abs(b[0]) + abs(b[1]), abs(b[2]), abs(b[4]), abs(b[6]), 1e-300)
+    out = {
+        "w": w, "g": g, "cfg": cfg, "state": state,
+        "config": {"workload": f"coastal_surge {args.cells}x{args.cells} dx=5m: ramp -5..+30 m + fractal "
+                               "noise, Manning map 0.02/0.06, surge maps every 600 s (device slots), "
+                               "hydrograph 0-500 cfs, tile acceleration on",
+                   "cells": args.cells, "tiles": g.ntiles, "spinup_to_s": SPINUP_S,
+                   "spinup_steps": spin_steps, "t_timed_start_s": w["t_start"],
+                   "active_tiles": w["n_act"], "active_tile_frac": w["n_act"] / g.ntiles,
+                   "wet_cell_frac_at_spinup": wet_frac,
+                   "wet_cells_per_step": w["wcu"] / max(args.steps, 1),
+                   "surge_window": [k0, k1, f], "surge_maps_missing": miss,
+                   "parallelism": "single GPU"},
+        "roofline": roofline(w, sb, traffic),
+        "step_roofline": {"bytes_per_wcu": sb["step"], "achieved_GBps": w["value"] * sb["step"] / 1e9,
+                          "frac": w["value"] * sb["step"] / HBM_PEAK},
+        "mass_budget": {"window": f"t = 0 .. {t_spin:.0f} s ({spin_steps} steps), FP64 on the device",
+                        "stored_m3": b[0], "source_in_m3": b[4], "outflow_m3": b[6],
+                        "surge_and_clamp_exchange_m3": b[7], "residual_m3": b[8],
+                        "residual_rel": abs(b[8]) / scale},
+    }
+    return out
+
+
+def rain_basin_leg(args):
+    """Rain basin 2048 x 2048 with every tile active (the rain keeps every
+    cell wet, so the active set fills from tile 0 within a few hundred
+    steps): K timed steps at full activity."""
+    g, cfg = build_rain_basin(2048)
+    steps = 0
+    while True:
+        g.step(50)
+        steps += 50
+        _, _, n = g.tiles()
+        if n == g.ntiles or steps >= 1000:
+            break
+    log(f"rain basin at full activity after {steps} steps ({n}/{g.ntiles} tiles)")
+    sb = STAGE_BYTES["rain_basin"]
+    w = timed_windows(g, max(args.steps, 50), 5, -1, NoRegions(), sb)
+    g.close()
+    return {"value": w["value"], "unit": "WCU/s", "ms_per_step": w["ms_per_step"],
+            "workload": "rain_basin 2048x2048 dx=10m, rain + Green-Ampt + Manning, every tile active",
+            "spinup_steps": steps, "active_tiles": w["n_act"],
+            "roofline": roofline(w, sb, None),
+            "step_roofline": {"bytes_per_wcu": sb["step"], "frac": w["value"] * sb["step"] / HBM_PEAK},
+            "kernel_ms_per_step": w["kernel_ms_per_step"], "launch_variant": w["variant"],
+            "variant_ms_per_step": w["variant_ms"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=160)
-    ap.add_argument("--cells", type=int, default=2048)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cells", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calibrate", action="store_true",
-                    help="also run one 256 MiB device-to-device copy (a known-size reference "
-                         "for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+                    help="also run one 256 MiB device-to-device copy in a profiled region (a "
+                         "known-size reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--rain-basin", type=int, default=1, help="run the 2048^2 full-activity leg (0: skip)")
     ap.add_argument("--ensemble", type=int, default=64,
                     help="scenarios of the batched-ensemble leg (0: skip)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="launch variant of the marching kernels (default: autotuned after the "
-                         "warm-up, outside the timed region)")
+                         "spin-up, outside the timed region)")
     ap.add_argument("--surrogate-train", type=int, default=512,
                     help="training scenarios of the surrogate-screening leg (0: skip the leg)")
     ap.add_argument("--surrogate-test", type=int, default=512,
@@ -266,6 +530,8 @@ def main():
               "(the solver runs one domain per GPU)",
               file=sys.stderr)
         sys.exit(2)
+    regions = Regions()
+    regions.pause()
 
     surr_data = None
     if args.surrogate_train > 0:
@@ -274,123 +540,62 @@ def main():
         from rift_amd import surrogate
         t0 = time.perf_counter()
         surr_data = surrogate.prepare(args.surrogate_train, args.surrogate_test, workers=8)
-        print(f"[bench] surrogate scenarios generated ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
-              flush=True)
+        log(f"surrogate scenarios generated ({time.perf_counter() - t0:.0f} s)")
 
-    g, cfg = build_workload(args.cells)
-    print(f"[bench] workload ready: {g.nx}x{g.ny} internal, {g.ntiles} tiles", file=sys.stderr,
-          flush=True)
-    g.step(args.warmup)
-    g.sync()
-    _, wcu0 = g.wet_cells()
-    wet, act, n_act = g.tiles()
-    print(f"[bench] warm-up done: {n_act}/{g.ntiles} tiles active", file=sys.stderr, flush=True)
-    variant_ms = None
-    if args.variant >= 0:
-        g.set_variant(args.variant)
-        variant = args.variant
-    else:
-        variant, variant_ms = g.autotune(64)
-        print(f"[bench] autotuned launch variant {variant}: "
-              + " ".join(f"{m:.4f}" for m in variant_ms) + " ms/step", file=sys.stderr, flush=True)
-
-    # timed region: K steps enqueued back to back, stream synchronised on both sides
-    g.sync()
-    t0 = time.perf_counter()
-    g.step(args.steps)
-    g.sync()
-    elapsed = time.perf_counter() - t0
-    per_step, wcu1 = g.wet_cells()
-    wcu = wcu1 - wcu0
-
-    # kernel durations: a further window with every launch bracketed by HIP
-    # events on the handle's stream (kept out of the wall-clock window above)
-    kp_steps = min(args.steps, 100)
-    g.profile(True)
-    _, wk0 = g.wet_cells()
-    g.step(kp_steps)
-    g.sync()
-    _, wk1 = g.wet_cells()
-    n1, ms1 = g.kernel_stats("flux1")
-    n2, ms2 = g.kernel_stats("stage2_flux")
-    ng, msg = g.kernel_stats("grow")
-    nt, mst = g.kernel_stats("timestep")
-    g.profile(False)
-    t_sim, dt = g.time
-    mb = g.mass_balance()
-    wet, act, n_act = g.tiles()
-
-    value = wcu / elapsed
-    ms_per_step = 1e3 * elapsed / args.steps
-    cells_per_step = (wk1 - wk0) / kp_steps
-    kern = {}
-    for name, n, ms in (("march1", n1, ms1), ("march2", n2, ms2)):
-        avg_s = ms / max(n, 1) * 1e-3
-        nbytes = cells_per_step * KERNEL_BYTES_PER_CELL[name]
-        kern[name] = {"avg_launch_ms": avg_s * 1e3, "bytes_per_launch": nbytes,
-                      "achieved_GBps": nbytes / avg_s / 1e9 if avg_s > 0 else 0.0}
-    dom = max(kern, key=lambda k: kern[k]["avg_launch_ms"])
-    achieved = kern[dom]["achieved_GBps"] * 1e9
-    traffic = measured_traffic("k_march<2>" if dom == "march2" else "k_march<1>", args.cells)
-    copy_bw = copy_bandwidth()
+    c = coastal_leg(args, regions)
+    w, g = c["w"], c["g"]
     line = {
         "metric": "wet-cell updates/s",
-        "value": value,
+        "value": w["value"],
         "unit": "WCU/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": w["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"rain_basin {args.cells}x{args.cells} dx=10m, rain+Green-Ampt+Manning, "
-                               "tile acceleration on", "cells": args.cells, "tiles": g.ntiles,
-                   "active_tiles": int(n_act), "wet_cells_per_step": wcu / args.steps,
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": traffic,
-                     "traffic_source": os.path.relpath(PROFILE, ROOT),
-                     "kernel": "k_march<2>" if dom == "march2" else "k_march<1>",
-                     "bytes_per_launch": kern[dom]["bytes_per_launch"],
-                     "avg_launch_ms": kern[dom]["avg_launch_ms"],
-                     "traffic_GBps": None if traffic is None
-                     else traffic / (kern[dom]["avg_launch_ms"] * 1e-3) / 1e9},
-        "hbm_copy": copy_bw,
-        "kernels": kern,
-        "step_roofline": {"bytes_per_wcu": STEP_BYTES_PER_WCU,
-                          "achieved_GBps": value * STEP_BYTES_PER_WCU / 1e9,
-                          "frac": value * STEP_BYTES_PER_WCU / HBM_PEAK},
-        "kernel_ms_per_step": {"march1": ms1 / kp_steps, "march2": ms2 / kp_steps,
-                               "grow": msg / kp_steps, "timestep": mst / kp_steps},
-        "launch_variant": variant,
-        "variant_ms_per_step": None if variant_ms is None else [float(m) for m in variant_ms],
-        "sim_time_s": t_sim,
-        "mass_balance_m3": {"stored": mb[0], "infiltrated": mb[1], "rain_in": mb[2]},
+        "config": c["config"],
+        "roofline": c["roofline"],
+        "step_roofline": c["step_roofline"],
+        "kernels": w["kern"],
+        "kernel_ms_per_step": w["kernel_ms_per_step"],
+        "launch_variant": w["variant"],
+        "variant_ms_per_step": None if w["variant_ms"] is None else [float(m) for m in w["variant_ms"]],
+        "variant_spread": None if w["variant_ms"] is None
+        else float(max(w["variant_ms"]) / min(w["variant_ms"]) - 1.0),
+        "mass_budget": c["mass_budget"],
+        "profile_state": c["state"],
     }
+    line["hbm_copy"] = copy_bandwidth()
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = coastal_cpu_baseline(c["cfg"], g)
+    else:
+        line["cpu_baseline"] = None
+    g.close()
+    del c, g
+    if args.rain_basin:
+        log("rain basin 2048 at full activity ...")
+        line["rain_basin_2048"] = rain_basin_leg(args)
     if args.ensemble > 0:
-        print("[bench] ensemble ...", file=sys.stderr, flush=True)
+        log("ensemble ...")
         line["ensemble"] = ensemble_leg(args.ensemble)
     else:
         line["ensemble"] = None
     if surr_data is not None:
-        print("[bench] surrogate screening ...", file=sys.stderr, flush=True)
+        log("surrogate screening ...")
         line["surrogate"] = surrogate_leg(surr_data, args.surrogate_epochs)
         del surr_data
     else:
         line["surrogate"] = None
-    if not args.no_cpu:
-        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(cfg, g)
-    else:
-        line["cpu_baseline"] = None
     if args.calibrate:
+        regions.resume()
         line["calibration_copy_bytes"] = calibration_copy()
-    g.close()
-    print(json.dumps(line))
+        regions.pause()
+    print(json.dumps(line, default=_json_default))
 
 
 if __name__ == "__main__":

@@ -29,6 +29,14 @@ int rift_geometry(int b_nx, int b_ny, const double* dem, const double* h0c, cons
                   double n_const, const double* Kc, double* BX, double* BY, double* h0,
                   double* n, double* K);
 
+/* GridSeries::p_interp (GridSeries.cu:91-122): the internal-grid map
+ * (cells, ny x nx doubles, nx = b_nx+3, ny = b_ny+3) of a forcing map given
+ * at the DEM corners (b_ny x b_nx, row 0 = south): the four-corner mean
+ * where more than two corners differ from nodata, times scale; every other
+ * cell holds nodata when allow_nodata, else 0. */
+int rift_map_cells(int b_nx, int b_ny, const double* corners, double scale, int allow_nodata,
+                   double nodata, double* cells);
+
 /* ---- File formats ------------------------------------------------------
  * CSV series, Source::ReadSource (source.cc:11-63): `hours,value` lines, '#'
  * comment lines, times converted to seconds in float, a zero-rate point

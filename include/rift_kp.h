@@ -144,6 +144,29 @@ int rift_interp_map(rift_grid* g, int which, double factor);
  * GridSeries.cu:345-378). */
 int rift_copy_map(rift_grid* g, int which, int dst, int src);
 
+/* Device-resident interpolated surge series: replaces InterpolatedGridSeries
+ * (GridSeries.cu:291-384) -- its two device planes, the per-step
+ * InterpGrid launch (grid.cu:313-318) and the host's p_update calls after
+ * every step (simulator.cu:827-829).  Map k (the spatially interpolated map
+ * of file index k, nx*ny doubles as p_interp leaves it, GridSeries.cu:91-
+ * 122) is held in slot k % nslots of device memory.  The fused step's dt
+ * kernel advances p_update's window at each new time on the device and the
+ * marching kernels and TrackSurge read the two planes of the window,
+ * interpolating per cell, so a step needs no host call and no full-grid
+ * interpolation pass.  nslots >= number of maps keeps the whole series
+ * resident; with fewer slots the caller loads map k before the window
+ * needs it (two slots suffice when maps are loaded as p_update reads them).
+ * Only which = RIFT_MAP_SURGE (handle created with RIFT_SURGE_GRID).
+ * Call order: rift_init (or rift_set_time), then rift_map_series_start. */
+int rift_map_series_create(rift_grid* g, int which, int nslots, int deltat, double tmax);
+int rift_map_series_load(rift_grid* g, int which, int index, const double* host);
+/* The first p_update at time t (GridSeries.cu:333-352): window maps
+ * trunc(t/deltat) and the next one. */
+int rift_map_series_start(rift_grid* g, int which, double t);
+/* The window in force (map indices, factor) and the first map index a
+ * window needed that was not loaded in its slot (-1: none); blocking. */
+int rift_map_series_state(rift_grid* g, int which, int* k0, int* k1, double* factor, int* missing);
+
 /* One or more complete time steps: the body of Simulator::RunSimulation
  * (simulator.cu:780-816) -- forcing update, Grow, two flux/integrate stages
  * (one in Euler mode), boundaries -- entirely on the device. */
@@ -174,6 +197,10 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
  * bit. */
 #define RIFT_OPT_VARIANT 3
 #define RIFT_NVARIANTS 16
+/* RIFT_OPT_MASS_BUDGET: 1 = step with the FP64 volume budget kernels (as
+ * RIFT_MASS_BUDGET at creation), 0 = without; switching it on resets the
+ * budget (rift_budget_reset).  The state arithmetic is the same either way. */
+#define RIFT_OPT_MASS_BUDGET 4
 int rift_set_option(rift_grid* g, int option, int value);
 
 /* Autotuner (no counterpart in the reference, whose 16 x 14 launch shape is

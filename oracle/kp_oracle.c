@@ -263,6 +263,7 @@ typedef struct series {
 
 typedef struct map_series {
     int configured, interpolated, allow_nodata, nmaps, deltat;
+    int first;             /* file index of maps[0] */
     double tmax, scale, in_time, next_time, sum_cache;
     int done, cur_init;
     double* maps;          /* nmaps * b_nx*b_ny */
@@ -351,8 +352,8 @@ void orc_apply_boundaries(orc_sim* s)
  * than two corners hold data, times the series scale. */
 static int ms_read(orc_sim* s, map_series* ms, int index)
 {
-    if (index < 0 || index >= ms->nmaps) return -1;   /* missing file */
-    const double* b = ms->maps + (size_t)index * (size_t)s->b_nx * (size_t)s->b_ny;
+    if (index < ms->first || index >= ms->first + ms->nmaps) return -1;   /* missing file */
+    const double* b = ms->maps + (size_t)(index - ms->first) * (size_t)s->b_nx * (size_t)s->b_ny;
     size_t N = (size_t)s->nx * (size_t)s->ny;
     real fill = ms->allow_nodata ? (real)s->nodata : (real)0;
     for (size_t k = 0; k < N; ++k) ms->ibuf[k] = fill;
@@ -490,6 +491,7 @@ static void ms_interp_update(orc_sim* s, map_series* ms, double t)
     }
     real factor = (real)((t - ms->in_time) / (ms->next_time - ms->in_time));
     const real nd = (real)s->nodata;
+#pragma omp parallel for schedule(static)
     for (int j = 2; j < s->ny; ++j) {
         for (int i = 2; i < s->nx; ++i) {
             size_t c = IX(s, i, j);
@@ -512,8 +514,15 @@ static void ms_update(orc_sim* s, int which, double t)
 
 int orc_set_map_series(orc_sim* s, int which, const double* maps, int nmaps, int deltat, double tmax)
 {
-    if (which < 0 || which > 2 || nmaps <= 0 || deltat <= 0) return -1;
+    return orc_set_map_series_range(s, which, 0, maps, nmaps, deltat, tmax);
+}
+
+int orc_set_map_series_range(orc_sim* s, int which, int first, const double* maps, int nmaps, int deltat,
+                             double tmax)
+{
+    if (which < 0 || which > 2 || nmaps <= 0 || deltat <= 0 || first < 0) return -1;
     map_series* ms = &s->ms[which];
+    ms->first = first;
     size_t per = (size_t)s->b_nx * (size_t)s->b_ny;
     size_t N = (size_t)s->nx * (size_t)s->ny;
     free(ms->maps);
@@ -640,35 +649,54 @@ static void rebuild_list(orc_sim* s)
         if (s->aval[t] > -1) s->list[s->nlist++] = t;
 }
 
+/* TrackRainfall_k / TrackSurge_k, grid.cu:1224-1276: a tile inactive in the
+ * last Grow (active value < 0; tile 0 never, -0 == 0) and dry becomes wet
+ * when a cell of its window holds gridded rain > epsilon (kind 0) or surge
+ * != nodata and > 0 (kind 1).  The window's rows use the BLOCK_COLS stride
+ * (j = by*16 + ty + 2, ty < 14) as the reference's do; cells outside
+ * [2, nx-3] x [2, ny-3] are skipped.  map: nx*ny values of the current map. */
+static void track_tiles(int nx, int ny, int nbx, int nby, const real* map, int kind, real nodata,
+                        const int* aval, unsigned char* wet)
+{
+    for (int tile = 0; tile < nbx * nby; ++tile) {
+        if (aval[tile] >= 0 || wet[tile]) continue;
+        int bx = tile % nbx, by = tile / nbx;
+        int hit = 0;
+        for (int ty = 0; ty < TILE_H && !hit; ++ty) {
+            for (int tx = 0; tx < TILE_W; ++tx) {
+                int i = bx * TILE_W + tx + 2;
+                int j = by * TILE_W + ty + 2;
+                if (i < 2 || j < 2 || i > nx - 3 || j > ny - 3) continue;
+                real v = map[(size_t)j * (size_t)nx + (size_t)i];
+                if (kind == 0 ? (v > EPS) : (v != nodata && v > 0)) { hit = 1; break; }
+            }
+        }
+        if (hit) wet[tile] = 1;
+    }
+}
+
+void orc_track_tiles(int nx, int ny, int nbx, int nby, const double* map, int kind, double nodata,
+                     const int* aval, unsigned char* wet)
+{
+    const size_t N = (size_t)nx * (size_t)ny;
+    real* m = (real*)malloc(sizeof(real) * (N ? N : 1));
+    for (size_t k = 0; k < N; ++k) m[k] = (real)map[k];
+    track_tiles(nx, ny, nbx, nby, m, kind, (real)nodata, aval, wet);
+    free(m);
+}
+
 /* Grow, grid.cu:1279-1322, with TrackRainfall_k/TrackSurge_k (1224-1276):
  * previously inactive, dry tiles become wet when gridded rain > epsilon or
  * surge > 0 in their cells; rows use the BLOCK_COLS stride of the reference. */
 void orc_grow(orc_sim* s)
 {
-    const int nx = s->nx, ny = s->ny;
     if (s->flags & (ORC_RAIN_GRID | ORC_SURGE_GRID)) {
         for (int pass = 0; pass < 2; ++pass) {
             if (pass == 0 && !(s->flags & ORC_RAIN_GRID)) continue;
             if (pass == 1 && !(s->flags & ORC_SURGE_GRID)) continue;
             const real* map = (pass == 0) ? s->ms[ORC_MAP_RAIN].cur : s->ms[ORC_MAP_SURGE].cur;
             if (!map) continue;
-            for (int k = 0; k < s->ntiles; ++k) {
-                if (s->aval[k] >= 0) continue;
-                int tile = -s->aval[k];
-                if (s->wet[tile]) continue;
-                int bx = tile % s->nbx, by = tile / s->nbx;
-                int hit = 0;
-                for (int ty = 0; ty < TILE_H && !hit; ++ty) {
-                    for (int tx = 0; tx < TILE_W; ++tx) {
-                        int i = bx * TILE_W + tx + 2;
-                        int j = by * TILE_W + ty + 2;
-                        if (i < 2 || j < 2 || i > nx - 3 || j > ny - 3) continue;
-                        real v = map[IX(s, i, j)];
-                        if (pass == 0 ? (v > EPS) : (v != (real)s->nodata && v > 0)) { hit = 1; break; }
-                    }
-                }
-                if (hit) s->wet[tile] = 1;
-            }
+            track_tiles(s->nx, s->ny, s->nbx, s->nby, map, pass, (real)s->nodata, s->aval, s->wet);
         }
     }
     orc_grow_tiles(s->nbx, s->nby, s->wet, s->aval);

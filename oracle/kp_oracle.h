@@ -94,6 +94,11 @@ double orc_kappa(double dx, double dy);
 /* Tile dilation of a wet pattern (Grow_k, grid.cu:1188-1222, race-free rule).
  * active_out[t] = t if active else -t. */
 void   orc_grow_tiles(int nbx, int nby, const unsigned char* wet, int* active_out);
+/* TrackRainfall_k (kind 0: rain > epsilon) / TrackSurge_k (kind 1: surge
+ * != nodata and > 0), grid.cu:1224-1276, on the tiles inactive in the last
+ * Grow (aval < 0) that are dry; map = nx*ny current map, wet updated. */
+void   orc_track_tiles(int nx, int ny, int nbx, int nby, const double* map, int kind, double nodata,
+                       const int* aval, unsigned char* wet);
 
 /* --- whole simulation ----------------------------------------------------- */
 /* dem: b_nx*b_ny raw DEM (row 0 = south).  h0/n/K corner maps may be NULL.
@@ -116,6 +121,11 @@ int      orc_set_sources(orc_sim* s, const int* idx, long n);
  * seconds (GridSeries.cuh:29).  which = orc_map_id. */
 int      orc_set_map_series(orc_sim* s, int which, const double* maps, int nmaps,
                             int deltat, double tmax);
+/* The same with the maps of file indices first .. first+nmaps-1 only (a run
+ * continued from a hotstart needs only the maps around its time); a map
+ * outside the range reads as a missing file. */
+int      orc_set_map_series_range(orc_sim* s, int which, int first, const double* maps, int nmaps,
+                                  int deltat, double tmax);
 
 /* One RunSimulation() call (simulator.cu:773-853). */
 void     orc_step(orc_sim* s);

@@ -54,7 +54,8 @@ __global__ void k_begin_step(Dev d)
 __global__ __launch_bounds__(256) void k_track(Dev d)
 {
     const int tid = threadIdx.x;
-    const bool rain = d.flags & RIFT_RAIN_GRID, surge = d.flags & RIFT_SURGE_GRID;
+    // a surge series on the device is tracked by k_track_series
+    const bool rain = d.flags & RIFT_RAIN_GRID, surge = (d.flags & RIFT_SURGE_GRID) && !d.sg_series;
     const float* rmap = d.map[RIFT_MAP_RAIN][0];
     const float* smap = d.map[RIFT_MAP_SURGE][0];
     for (int t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
@@ -76,6 +77,79 @@ __global__ __launch_bounds__(256) void k_track(Dev d)
         if (__syncthreads_or(hit) && tid == 0) d.wet[t] = 1;
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// Device surge series.  A slot's tile flags say where its map can wake a
+// tile or override a cell at all: InterpGrid_k's value is a convex
+// combination of the two planes (or one of them), and f*(x1-x0)+x0 rounds to
+// a value <= 0 whenever neither plane holds a value > 0 at the cell (both
+// planes <= 0: |f (x1-x0)| <= |x0| after rounding, which is monotone).  So a
+// tile whose flags are clear in both slots of a window has no cell with
+// surge > 0, and skipping it is exact.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sg_flags(Dev d, int slot)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.ntiles) return;
+    const float* m = d.sgs + (size_t)slot * (size_t)d.soff;
+    const int bx = t % d.nbx, by = t / d.nbx;
+    const float nd = d.nodata;
+    unsigned char bits = 0;
+    const int i0 = bx * kTileW + 2, i1 = min(i0 + kTileW - 1, d.nx - 3);
+    // TrackSurge_k's window: rows by*BLOCK_COLS + ty + 2, ty < BLOCK_ROWS
+    for (int j = by * kTileW + 2; j <= min(by * kTileW + kTileH + 1, d.ny - 3); ++j)
+        for (int i = i0; i <= i1; ++i) {
+            const float v = m[cell(d, i, j)];
+            if (v != nd && v > 0.f) bits |= 1;
+        }
+    // the tile's own cells (the surge override of Integrate_1_k)
+    for (int j = by * kTileH + 2; j <= min(by * kTileH + kTileH + 1, d.ny - 3); ++j)
+        for (int i = i0; i <= i1; ++i) {
+            const float v = m[cell(d, i, j)];
+            if (v != nd && v > 0.f) bits |= 2;
+        }
+    d.sgflag[(size_t)slot * d.ntiles + t] = bits;
+}
+
+// TrackSurge_k (grid.cu:1250-1276) over the device series: one lane per
+// tile; a dry tile that was inactive in the last Grow and whose slots can
+// hold surge > 0 in its Track window evaluates the window cell by cell.
+__global__ __launch_bounds__(256) void k_track_series(Dev d)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.ntiles || d.act[t] || d.wet[t]) return;
+    const SurgeWindow w = surge_window_now(d.sc);
+    const int s0 = sg_slot(d, w.k0), s1 = sg_slot(d, w.k1);
+    if (!((d.sgflag[(size_t)s0 * d.ntiles + t] | d.sgflag[(size_t)s1 * d.ntiles + t]) & 1)) return;
+    const float* m0 = d.sgs + (size_t)s0 * (size_t)d.soff;
+    const float* m1 = d.sgs + (size_t)s1 * (size_t)d.soff;
+    const int bx = t % d.nbx, by = t / d.nbx;
+    const int i0 = bx * kTileW + 2, i1 = min(i0 + kTileW - 1, d.nx - 3);
+    for (int j = by * kTileW + 2; j <= min(by * kTileW + kTileH + 1, d.ny - 3); ++j)
+        for (int i = i0; i <= i1; ++i) {
+            const size_t q = cell(d, i, j);
+            const float v = surge_interp(m0[q], m1[q], w.f, d.nodata);
+            if (v != d.nodata && v > 0.f) {
+                d.wet[t] = 1;
+                return;
+            }
+        }
+}
+
+// The window in force written into surge plane 0 over i in [2, nx-1],
+// j in [2, ny-1] (InterpGrid_k's range): what the per-phase kernels and the
+// field downloads read.
+__global__ __launch_bounds__(256) void k_sg_materialize(Dev d)
+{
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15) + 2;
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4) + 2;
+    if (i >= d.nx || j >= d.ny) return;
+    const SurgeWindow w = surge_window_now(d.sc);
+    const size_t q = cell(d, i, j);
+    const float* m0 = d.sgs + (size_t)sg_slot(d, w.k0) * (size_t)d.soff;
+    const float* m1 = d.sgs + (size_t)sg_slot(d, w.k1) * (size_t)d.soff;
+    d.map[RIFT_MAP_SURGE][0][q] = surge_interp(m0[q], m1[q], w.f, d.nodata);
 }
 
 // ---------------------------------------------------------------------------
@@ -542,8 +616,28 @@ void launch_begin_step(const Dev& d, hipStream_t s)
 
 void launch_track(const Dev& d, hipStream_t s)
 {
-    int grid = d.ntiles < 8192 ? d.ntiles : 8192;
-    hipLaunchKernelGGL(k_track, dim3(grid), dim3(256), 0, s, d);
+    const bool series = (d.flags & RIFT_SURGE_GRID) && d.sg_series;
+    if ((d.flags & RIFT_RAIN_GRID) || ((d.flags & RIFT_SURGE_GRID) && !series)) {
+        int grid = d.ntiles < 8192 ? d.ntiles : 8192;
+        hipLaunchKernelGGL(k_track, dim3(grid), dim3(256), 0, s, d);
+    }
+    if (series) launch_track_series(d, s);
+}
+
+void launch_track_series(const Dev& d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_track_series, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, d);
+}
+
+void launch_sg_flags(const Dev& d, int slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sg_flags, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, d, slot);
+}
+
+void launch_sg_materialize(const Dev& d, hipStream_t s)
+{
+    dim3 grid((d.nx + 15) / 16, (d.ny + 15) / 16);
+    hipLaunchKernelGGL(k_sg_materialize, grid, dim3(256), 0, s, d);
 }
 
 void launch_reset_counters(const Dev& d, hipStream_t s)

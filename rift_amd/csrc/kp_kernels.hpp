@@ -58,6 +58,24 @@ struct Scalars {
     double b_rain_grid, b_drain, b_out, b_exch, b_flux1;
     int finished;                     // the domain reached its end time (ensembles)
     float nxt_rain, nxt_drain, nxt_hydro, nxt_cell;
+    // device surge series (InterpolatedGridSeries, GridSeries.cu:291-384):
+    // window state advanced by the dt kernel (p_in_time, p_next_time, p_done
+    // and the map indices of its two planes), the window it evaluated for
+    // the next step (valid flag, indices, factor), the window of the step
+    // being run (installed by grow2), and the first map index a window
+    // needed that was not loaded into its slot (-1: none)
+    double sg_in, sg_next;
+    int sg_done, sg_nk0, sg_nk1, sg_next_valid;
+    float sg_nf;
+    int sg_k0, sg_k1;
+    float sg_f;
+    int sg_missing;
+};
+
+// Surge window of one step: map indices of the two planes and the factor.
+struct SurgeWindow {
+    int k0, k1;
+    float f;
 };
 
 struct Dev {
@@ -88,7 +106,83 @@ struct Dev {
     unsigned flags;
     int si0, si1, sj0, sj1;      // hydrograph source rectangle (inclusive)
     long nsrc;
+    // device surge series: map k lives in slot k % sg_nslots, a plane of
+    // the field layout (slot s at sgs + s * soff); sgflag[s * ntiles + t]
+    // bit 0 = a cell of tile t's Track window (rows by*16+2 .. by*16+15,
+    // grid.cu:1265) holds a value > 0 (not nodata), bit 1 = a cell of the
+    // tile itself does; sgtag[s] = map index held by slot s (-1: none)
+    float* sgs;
+    unsigned char* sgflag;
+    int* sgtag;
+    int sg_series;               // 1: surge from the device series
+    int sg_nslots;
+    double sg_dt, sg_tmax;       // map interval (integer seconds) and surge_tf
 };
+
+// Slot of map k.
+__host__ __device__ inline int sg_slot(const Dev& d, int k)
+{
+    return ((k % d.sg_nslots) + d.sg_nslots) % d.sg_nslots;
+}
+
+// The surge window in force now: the one the last dt kernel evaluated for
+// the coming step, or (once grow2 installed it) the one of the running step.
+__device__ inline SurgeWindow surge_window_now(const Scalars* sc)
+{
+    SurgeWindow w;
+    if (sc->sg_next_valid) {
+        w.k0 = sc->sg_nk0; w.k1 = sc->sg_nk1; w.f = sc->sg_nf;
+    } else {
+        w.k0 = sc->sg_k0; w.k1 = sc->sg_k1; w.f = sc->sg_f;
+    }
+    return w;
+}
+
+// InterpGrid_k's nodata rule and interpolant (grid.cu:292-311) in FP32, as
+// k_interp_map evaluates it (no contraction).
+__device__ __forceinline__ float surge_interp(float x0, float x1, float f, float nd)
+{
+    if (x1 == nd && x0 == nd) return nd;
+    if (x1 == nd) return x0;
+    if (x0 == nd) return x1;
+    return f * (x1 - x0) + x0;
+}
+
+// InterpolatedGridSeries::p_update (GridSeries.cu:325-384) at the new time
+// sc->t, after the first update: the window moves to the next pair of maps
+// once t reaches p_next_time, freezes on the last map once t reaches
+// surge_tf, and the factor (t - p_in_time)/(p_next_time - p_in_time) is
+// formed in double as there.  The result is the next step's window.
+__device__ inline void advance_surge_window(const Dev& d, Scalars* sc)
+{
+    const double t = sc->t;
+    if (t >= d.sg_tmax) {
+        if (!sc->sg_done) {
+            sc->sg_in = sc->sg_next;
+            sc->sg_next = sc->sg_in + d.sg_dt;
+            sc->sg_nk0 = sc->sg_nk1;   // cudaMemcpy2D t1 -> t0
+        }
+        sc->sg_done = 1;
+    } else if (t >= sc->sg_next) {
+        sc->sg_nk0 = sc->sg_nk1;
+        sc->sg_in = sc->sg_next;
+        sc->sg_next = sc->sg_in + d.sg_dt;
+        sc->sg_nk1 = (int)trunc(sc->sg_next / d.sg_dt);
+    }
+    const double factor = (t - sc->sg_in) / (sc->sg_next - sc->sg_in);
+    sc->sg_nf = (float)factor;
+    sc->sg_next_valid = 1;
+}
+
+// Records the first map index of the installed window whose slot does not
+// hold it (the caller loads maps between steps, so this is checked when a
+// window comes into force, not when the dt kernel computes it).
+__device__ inline void check_surge_slots(const Dev& d, Scalars* sc)
+{
+    if (sc->sg_missing >= 0) return;
+    if (d.sgtag[sg_slot(d, sc->sg_k0)] != sc->sg_k0) sc->sg_missing = sc->sg_k0;
+    else if (d.sgtag[sg_slot(d, sc->sg_k1)] != sc->sg_k1) sc->sg_missing = sc->sg_k1;
+}
 
 __host__ __device__ inline size_t cell(const Dev& d, int i, int j)
 {
@@ -163,6 +257,8 @@ __device__ inline void finish_timestep(const Dev& d, float vx, float vy)
     const double cells = (double)sc->wet_cells;
     if (d.flags & RIFT_RAIN_AVG) sc->v_rain += (double)sc->rain_rate * dt * cells;
     if (d.flags & RIFT_HYDROGRAPH) sc->v_src += (double)sc->src_cell_rate * dt * (double)d.nsrc;
+    // surge_series->update(t) at the end of RunSimulation (simulator.cu:827-829)
+    if (d.sg_series) advance_surge_window(d, sc);
 }
 
 // Launchers (kp_kernels.hip).  All run on `s` and read t/dt/counts from d.sc.
@@ -176,6 +272,11 @@ void launch_integrate1(const Dev& d, hipStream_t s, int grid);
 void launch_integrate2(const Dev& d, hipStream_t s, int grid);
 void launch_boundaries(const Dev& d, hipStream_t s);
 void launch_interp_map(const Dev& d, int which, float factor, hipStream_t s);
+// device surge series: tile flags of one slot, Track of the series, and the
+// window in force written into map plane 0 (per-phase path, downloads)
+void launch_sg_flags(const Dev& d, int slot, hipStream_t s);
+void launch_track_series(const Dev& d, hipStream_t s);
+void launch_sg_materialize(const Dev& d, hipStream_t s);
 void launch_sum_interior(const Dev& d, const float* x, int mode, double* out, hipStream_t s);
 
 // Fused step (kp_step2.hip).  One wavefront marches one unit: 60 output

@@ -95,6 +95,14 @@ __device__ __forceinline__ void grow2_body(const Dev& d, int t)
             sc->src_cell_rate = sc->nxt_cell;
             sc->next_valid = 0;
         }
+        // the surge window of this step (k_track_series read it already)
+        if (sc->sg_next_valid) {
+            sc->sg_k0 = sc->sg_nk0;
+            sc->sg_k1 = sc->sg_nk1;
+            sc->sg_f = sc->sg_nf;
+            sc->sg_next_valid = 0;
+            if (d.sg_series) check_surge_slots(d, sc);
+        }
     }
     const bool is_tile = t < d.ntiles;
     const bool is_unit = t >= d.ntiles && t < d.ntiles + d.nunits;
@@ -231,8 +239,31 @@ __device__ __forceinline__ unsigned lane_off(unsigned x)
     return x;
 }
 
+// Scalars of a marching launch; sg0/sg1/sgf: the two surge planes of the
+// step's window and its factor (device surge series).
+struct StepScalars {
+    float dt, inv_dt, dtp, inv_dtp, kappa, rain_rate, drain_rate, src_rate, tnow, inv_dx, inv_dy;
+    unsigned fl;
+    const float* sg0;
+    const float* sg1;
+    float sgf;
+};
+
+// The surge planes of the running step's window (installed by grow2).
+__device__ __forceinline__ void surge_scalars(const Dev& d, StepScalars& S)
+{
+    S.sg0 = S.sg1 = nullptr;
+    S.sgf = 0.f;
+    if (d.sg_series) {
+        const Scalars* sc = d.sc;
+        S.sg0 = d.sgs + (size_t)sg_slot(d, sc->sg_k0) * (size_t)d.soff;
+        S.sg1 = d.sgs + (size_t)sg_slot(d, sc->sg_k1) * (size_t)d.soff;
+        S.sgf = sc->sg_f;
+    }
+}
+
 template <int Stage, unsigned FM>
-__device__ __forceinline__ Raw fetch(const Dev& d, const Lane& L, int j)
+__device__ __forceinline__ Raw fetch(const Dev& d, const StepScalars& S, const Lane& L, int j)
 {
     const int sr = min(max(j, 2), d.ny - 3);
     const int slot = tile_row_slot(L, sr);
@@ -249,7 +280,12 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const Lane& L, int j)
         r.dh = ld(d.dh + rb, sc4);
         r.dhu = ld(d.dhu + rb, sc4);
         r.dhv = ld(d.dhv + rb, sc4);
-        r.sg = (d.flags & FM & RIFT_SURGE_GRID) ? ld(d.map[RIFT_MAP_SURGE][0] + rb, sc4) : 0.f;
+        if (d.flags & FM & RIFT_SURGE_GRID) {
+            if (d.sg_series) r.sg = surge_interp(ld(S.sg0 + rb, sc4), ld(S.sg1 + rb, sc4), S.sgf, d.nodata);
+            else r.sg = ld(d.map[RIFT_MAP_SURGE][0] + rb, sc4);
+        } else {
+            r.sg = 0.f;
+        }
         r.n = d.n_uniform ? d.n_const : ld(d.n + rb, sc4);
     }
     return r;
@@ -351,11 +387,6 @@ __device__ __forceinline__ void shift_lane(const Dev& d, Lane& L, int rr)
     L.actN = d.act[TN] != 0;
 }
 
-// Scalars of a marching launch.
-struct StepScalars {
-    float dt, inv_dt, dtp, inv_dtp, kappa, rain_rate, drain_rate, src_rate, tnow, inv_dx, inv_dy;
-    unsigned fl;
-};
 
 // Per-tile results of the rows of tile row rr a wave marched in strip s:
 // speed maxima (stage 1 into mx1/my1 for this step's dt, stage 2 into mx/my,
@@ -570,11 +601,11 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
     float BY[3];       // BY of a row = the bed of its south face
 
     // prologue (phase 0): rows j-2 .. j+2 (both directions)
-    const Row A = finish<Stage, FM>(d, L, fetch<Stage, FM>(d, L, j - 2 * step), j - 2 * step, dt, inv_dt);
-    const Raw rB = fetch<Stage, FM>(d, L, j - step);
-    const Raw rC = fetch<Stage, FM>(d, L, j);
-    const Raw rD = fetch<Stage, FM>(d, L, j + step);
-    W[2] = fetch<Stage, FM>(d, L, j + 2 * step);
+    const Row A = finish<Stage, FM>(d, L, fetch<Stage, FM>(d, S, L, j - 2 * step), j - 2 * step, dt, inv_dt);
+    const Raw rB = fetch<Stage, FM>(d, S, L, j - step);
+    const Raw rC = fetch<Stage, FM>(d, S, L, j);
+    const Raw rD = fetch<Stage, FM>(d, S, L, j + step);
+    W[2] = fetch<Stage, FM>(d, S, L, j + 2 * step);
     SR[0] = fetch_src<Stage, FM>(d, L, j);
     const Row B = finish<Stage, FM>(d, L, rB, j - step, dt, inv_dt);
     R[0] = finish<Stage, FM>(d, L, rC, j, dt, inv_dt);
@@ -621,7 +652,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         // prefetch: the row after next and the next output row's inputs
         // (clamped to the domain; unused past the end of the run)
         const int jn = Up ? max(j - 3, 0) : j + 3;
-        const Raw rN = fetch<Stage, FM>(d, L, jn);
+        const Raw rN = fetch<Stage, FM>(d, S, L, jn);
         SR[kd] = fetch_src<Stage, FM>(d, L, min(max(j + step, 2), d.ny - 3));
 
         const float by_behind = BY[ke];    // BY of row j - step
@@ -759,6 +790,7 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
     S.tnow = (float)d.sc->t;
     S.inv_dx = d.inv_dx;
     S.inv_dy = d.inv_dy;
+    surge_scalars(d, S);
 
     const int rows = d.ny - 4;   // interior rows of a strip
     const int mlim = max(1, rows / max(d.min_rows, 1));
@@ -1074,6 +1106,7 @@ __device__ __forceinline__ StepScalars step_scalars(const Dev& d)
     S.tnow = (float)d.sc->t;
     S.inv_dx = d.inv_dx;
     S.inv_dy = d.inv_dy;
+    surge_scalars(d, S);
     return S;
 }
 

@@ -80,6 +80,7 @@ struct rift_grid {
     hipGraphExec_t graph1 = nullptr, graphN = nullptr;
     bool use_graphs = true;
     bool prof = false;
+    int sg_dt_int = 0;            // device surge series: map interval, seconds
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
     std::vector<hipEvent_t> event_pool;
 };
@@ -694,6 +695,8 @@ int rift_step(rift_grid* g, int nsteps)
         int rc;
         if (g->d.flags & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG))
             launch_begin_step(g->d, g->stream);
+        // the per-phase kernels read the surge of this step from map plane 0
+        if ((g->d.flags & RIFT_SURGE_GRID) && g->d.sg_series) launch_sg_materialize(g->d, g->stream);
         if ((rc = rift_grow(g))) return rc;
         if ((rc = rift_compute_fluxes(g))) return rc;
         if ((rc = rift_compute_timestep(g))) return rc;
@@ -723,6 +726,25 @@ int rift_set_option(rift_grid* g, int option, int value)
     case RIFT_OPT_GRAPHS:
         g->use_graphs = value != 0;
         return RIFT_OK;
+    case RIFT_OPT_MASS_BUDGET: {
+        // the budget instantiation of the marching kernels on or off; the
+        // accumulators restart from the current state when it comes on
+        const bool on = value != 0;
+        if (on == ((g->d.flags & RIFT_MASS_BUDGET) != 0)) return RIFT_OK;
+        HIPCHK(hipStreamSynchronize(g->stream));
+        drop_graphs(g);
+        if (on) g->d.flags |= RIFT_MASS_BUDGET;
+        else g->d.flags &= ~RIFT_MASS_BUDGET;
+        g->desc.flags = g->d.flags;
+        for (int st = 0; st < 2; ++st)
+            for (int v = 0; v < kMarchVariants; ++v)
+                g->march_blocks[st][v] = march_resident_blocks(st + 1, v, g->d.flags, g->cus);
+        if (on && g->inited) {
+            int rc = rift_budget_reset(g);
+            if (rc) return rc;
+        }
+        return prepare_graphs(g);
+    }
     case RIFT_OPT_VARIANT:
         if (value < 0 || value >= RIFT_NVARIANTS) return fail(RIFT_E_ARG, "rift_set_option: no such variant");
         drop_graphs(g);   // the graphs hold the launch configuration
@@ -770,6 +792,11 @@ int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
     rift_grid* c = nullptr;
     int rc = rift_create(&g->desc, &c);
     if (rc) return rc;
+    if (g->d.sg_series &&
+        (rc = rift_map_series_create(c, RIFT_MAP_SURGE, g->d.sg_nslots, g->sg_dt_int, g->d.sg_tmax))) {
+        rift_destroy(c);
+        return rc;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double t_best = 0.0;
     int v_best = -1;
@@ -804,6 +831,102 @@ int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
     if (rc) return rc;
     *best = v_best;
     return rift_set_option(g, RIFT_OPT_VARIANT, v_best);
+}
+
+int rift_map_series_create(rift_grid* g, int which, int nslots, int deltat, double tmax)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_map_series_create: null handle");
+    if (which != RIFT_MAP_SURGE)
+        return fail(RIFT_E_UNSUPPORTED, "rift_map_series_create: only the surge series lives on the device");
+    if (!(g->d.flags & RIFT_SURGE_GRID))
+        return fail(RIFT_E_STATE, "rift_map_series_create: handle created without RIFT_SURGE_GRID");
+    if (g->d.sg_series) return fail(RIFT_E_STATE, "rift_map_series_create: series exists");
+    if (nslots < 2 || deltat <= 0) return fail(RIFT_E_ARG, "rift_map_series_create: nslots >= 2, deltat > 0");
+    Dev& d = g->d;
+    void* p = nullptr;
+    const size_t bytes = (size_t)nslots * g->plane * sizeof(float);
+    HIPCHK(hipMalloc(&p, bytes));
+    g->allocs.push_back(p);
+    g->alloc_sizes.push_back(bytes);
+    HIPCHK(hipMemsetAsync(p, 0, bytes, g->stream));
+    d.sgs = static_cast<float*>(p);
+    d.sgflag = alloc_bytes<unsigned char>(g, (size_t)nslots * d.ntiles);
+    d.sgtag = alloc_bytes<int>(g, nslots);
+    if (!d.sgflag || !d.sgtag) return fail(RIFT_E_HIP, "rift_map_series_create: device allocation failed");
+    std::vector<int> tags(nslots, -1);
+    HIPCHK(hipMemcpyAsync(d.sgtag, tags.data(), nslots * sizeof(int), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    // plane 0 (what downloads and the per-phase kernels read) starts at
+    // nodata, as p_init_dev fills the series' planes (GridSeries.cu:226-231)
+    {
+        std::vector<double> fill((size_t)d.nx * d.ny, d.nodata);
+        int rc = put_field(g, d.map[RIFT_MAP_SURGE][0], fill.data(), d.nx, d.ny, d.nx);
+        if (rc) return rc;
+    }
+    d.sg_nslots = nslots;
+    d.sg_dt = (double)deltat;
+    d.sg_tmax = tmax;
+    d.sg_series = 1;
+    g->sg_dt_int = deltat;
+    g->primed = false;
+    drop_graphs(g);
+    return RIFT_OK;
+}
+
+int rift_map_series_load(rift_grid* g, int which, int index, const double* host)
+{
+    if (!g || !host) return fail(RIFT_E_ARG, "rift_map_series_load: null argument");
+    if (which != RIFT_MAP_SURGE || !g->d.sg_series)
+        return fail(RIFT_E_STATE, "rift_map_series_load: no device series for this map");
+    if (index < 0) return fail(RIFT_E_ARG, "rift_map_series_load: negative map index");
+    Dev& d = g->d;
+    const int slot = sg_slot(d, index);
+    int rc = put_field(g, d.sgs + (size_t)slot * g->plane, host, d.nx, d.ny, d.nx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(d.sgtag + slot, &index, sizeof(int), hipMemcpyHostToDevice, g->stream));
+    launch_sg_flags(d, slot, g->stream);
+    if ((rc = check_launch("rift_map_series_load"))) return rc;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_map_series_start(rift_grid* g, int which, double t)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_map_series_start: null handle");
+    if (which != RIFT_MAP_SURGE || !g->d.sg_series)
+        return fail(RIFT_E_STATE, "rift_map_series_start: no device series for this map");
+    Scalars sc;
+    HIPCHK(hipStreamSynchronize(g->stream));
+    HIPCHK(hipMemcpy(&sc, g->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    // first InterpolatedGridSeries::p_update (GridSeries.cu:333-352)
+    const int index = (int)std::trunc(t / (double)g->sg_dt_int);
+    sc.sg_in = (double)(index * g->sg_dt_int);
+    sc.sg_next = sc.sg_in + g->sg_dt_int;
+    sc.sg_done = 0;
+    sc.sg_nk0 = sc.sg_k0 = index;
+    sc.sg_nk1 = sc.sg_k1 = index + 1;
+    const double factor = (t - sc.sg_in) / (sc.sg_next - sc.sg_in);
+    sc.sg_nf = sc.sg_f = (float)factor;
+    sc.sg_next_valid = 1;
+    sc.sg_missing = -1;
+    HIPCHK(hipMemcpy(g->d.sc, &sc, sizeof(Scalars), hipMemcpyHostToDevice));
+    return RIFT_OK;
+}
+
+int rift_map_series_state(rift_grid* g, int which, int* k0, int* k1, double* factor, int* missing)
+{
+    if (!g) return fail(RIFT_E_ARG, "rift_map_series_state: null handle");
+    if (which != RIFT_MAP_SURGE || !g->d.sg_series)
+        return fail(RIFT_E_STATE, "rift_map_series_state: no device series for this map");
+    Scalars sc;
+    HIPCHK(hipMemcpyAsync(&sc, g->d.sc, sizeof(Scalars), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    const bool nv = sc.sg_next_valid != 0;
+    if (k0) *k0 = nv ? sc.sg_nk0 : sc.sg_k0;
+    if (k1) *k1 = nv ? sc.sg_nk1 : sc.sg_k1;
+    if (factor) *factor = nv ? sc.sg_nf : sc.sg_f;
+    if (missing) *missing = sc.sg_missing;
+    return RIFT_OK;
 }
 
 int rift_sync(rift_grid* g)
@@ -911,6 +1034,11 @@ int rift_download(rift_grid* g, int field, double* host)
                 host[(size_t)j * nx + i] = (double)(h * std::sqrt(v * v + u * u));
             }
         return RIFT_OK;
+    }
+    if (field == RIFT_SURGE && d.sg_series) {
+        // the series' window in force, as the reference's current surge map
+        launch_sg_materialize(d, g->stream);
+        if ((rc = check_launch("rift_download"))) return rc;
     }
     float* p = field_plane(g, field);
     if (!p) return fail(RIFT_E_ARG, "rift_download: field not available with these flags");

@@ -1,5 +1,6 @@
 // geometry.cpp -- internal-grid construction of Simulator::InitSimulation
-// (simulator.cu:267-436) and the kappa formula (simulator.cu:281).
+// (simulator.cu:267-436), the kappa formula (simulator.cu:281) and the
+// corner-to-cell rule of the gridded forcing series (GridSeries.cu:91-122).
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -50,6 +51,40 @@ extern "C" int rift_geometry(int b_nx, int b_ny, const double* dem, const double
             if (K && Kc) {
                 double k = 0.25f * (Kc[ur] + Kc[lr] + Kc[ll] + Kc[ul]);
                 K[c] = k / (3600.f * 1000.f);
+            }
+        }
+    }
+    return RIFT_OK;
+}
+
+// GridSeries::p_interp (GridSeries.cu:91-122): a cell takes the mean of its
+// four corner values when more than two of them hold data, times the series
+// scale; every other cell (ghosts included) holds the fill value, nodata
+// when the series allows it, else 0.
+extern "C" int rift_map_cells(int b_nx, int b_ny, const double* corners, double scale,
+                              int allow_nodata, double nodata, double* cells)
+{
+    if (b_nx < 2 || b_ny < 2 || !corners || !cells) return RIFT_E_ARG;
+    const int nx = b_nx + 3, ny = b_ny + 3;
+    const double fill = allow_nodata ? nodata : 0.0;
+    for (size_t k = 0; k < (size_t)nx * ny; ++k) cells[k] = fill;
+    for (int j = 2; j < ny - 2; ++j) {
+        for (int i = 2; i < nx - 2; ++i) {
+            const int jt = j - 2, it = i - 2;
+            int nnd = 0;
+            double vsum = 0.0;
+            for (int ni = 0; ni < 2; ++ni)
+                for (int nj = 0; nj < 2; ++nj) {
+                    const double v = corners[(size_t)(jt + nj) * b_nx + it + ni];
+                    if (v != nodata) {
+                        ++nnd;
+                        vsum += v;
+                    }
+                }
+            if (nnd > 2) {
+                double v = vsum / (double)nnd;
+                v *= scale;
+                cells[(size_t)j * nx + i] = v;
             }
         }
     }

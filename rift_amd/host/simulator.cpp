@@ -24,6 +24,14 @@ MapSeries::MapSeries(rift_grid* g, int which, std::string prefix, double scale, 
       ibuf_((size_t)gc.h_nx * gc.h_ny, 0.0)
 {
     if (deltat_ <= 0) throw IoError(prefix_ + ": map series time step must be a positive integer");
+    // the surge series lives on the device: two slots, maps loaded as
+    // p_update reads them
+    device_ = interpolated_ && which_ == RIFT_MAP_SURGE;
+    if (device_) {
+        if (rift_map_series_create(g_, which_, 2, deltat_, tmax_) != RIFT_OK)
+            throw IoError(std::string("map series: ") + rift_last_error());
+        return;
+    }
     // p_init_dev: planes start at the fill value (nodata when allowed)
     std::vector<double> fill(ibuf_.size(), allow_nodata_ ? GridConfig::kNodata : 0.0);
     for (int plane = 0; plane < 3; ++plane)
@@ -39,26 +47,17 @@ void MapSeries::read_map(int index)
     Raster r = read_raster(name.str(), allow_nodata_, GridConfig::kNodata);
     if (r.ncols != gc_.b_nx || r.nrows != gc_.b_ny)
         throw IoError(name.str() + ": incorrect size");
-    std::fill(ibuf_.begin(), ibuf_.end(), allow_nodata_ ? GridConfig::kNodata : 0.0);
-    const int nx = gc_.h_nx, ny = gc_.h_ny, bnx = gc_.b_nx;
-    for (int j = 2; j < ny - 2; ++j) {
-        for (int i = 2; i < nx - 2; ++i) {
-            const int jt = j - 2, it = i - 2;
-            int nnd = 0;
-            double vsum = 0.0;
-            for (int ni = 0; ni < 2; ++ni)
-                for (int nj = 0; nj < 2; ++nj) {
-                    const double v = r.data[(size_t)(jt + nj) * bnx + it + ni];
-                    if (v != GridConfig::kNodata) { ++nnd; vsum += v; }
-                }
-            if (nnd > 2) {
-                double v = vsum / (double)nnd;
-                v *= scale_;
-                ibuf_[(size_t)j * nx + i] = v;
-            }
-        }
-    }
+    rift_map_cells(gc_.b_nx, gc_.b_ny, r.data.data(), scale_, allow_nodata_ ? 1 : 0, GridConfig::kNodata,
+                   ibuf_.data());
     std::cout << "Reading from " << name.str() << " ..." << std::endl;
+}
+
+// Map `index` into its slot of the device series.
+void MapSeries::load_device(int index)
+{
+    read_map(index);
+    if (rift_map_series_load(g_, which_, index, ibuf_.data()) != RIFT_OK)
+        throw IoError(std::string("map series load: ") + rift_last_error());
 }
 
 void MapSeries::upload(int plane)
@@ -108,6 +107,34 @@ void MapSeries::held_update(double t)
     }
 }
 
+// InterpolatedGridSeries::p_update (GridSeries.cu:325-384) with the device
+// series: the device advances the window and interpolates at every step; the
+// host keeps the same window only to read each map when p_update would,
+// into the slot of the two-slot series that the window has just left.
+void MapSeries::device_update(double t)
+{
+    if (in_time_ < 0.0) {
+        in_time_ = t;
+        const int index = (int)std::trunc(in_time_ / deltat_);
+        load_device(index);
+        in_time_ = index * (double)deltat_;
+        next_time_ = in_time_ + deltat_;
+        load_device(index + 1);
+        if (rift_map_series_start(g_, which_, t) != RIFT_OK)
+            throw IoError(std::string("map series start: ") + rift_last_error());
+    } else if (t >= tmax_) {
+        if (!done_) {
+            in_time_ = next_time_;
+            next_time_ = in_time_ + deltat_;
+        }
+        done_ = true;
+    } else if (t >= next_time_) {
+        in_time_ = next_time_;
+        next_time_ = in_time_ + deltat_;
+        load_device((int)std::trunc(next_time_ / deltat_));
+    }
+}
+
 // InterpolatedGridSeries::p_update (GridSeries.cu:325-384)
 void MapSeries::interp_update(double t)
 {
@@ -140,7 +167,8 @@ void MapSeries::interp_update(double t)
 
 void MapSeries::update(double t)
 {
-    if (interpolated_) interp_update(t);
+    if (device_) device_update(t);
+    else if (interpolated_) interp_update(t);
     else held_update(t);
 }
 
@@ -333,11 +361,6 @@ void Simulator::InitSimulation()
                                           drain_tf_, gc_, false, false));
         drain_series_->update(t0);
     }
-    if (surge_grid_) {
-        surge_series_.reset(new MapSeries(grid_, RIFT_MAP_SURGE, surge_prefix_, 1.0, (int)surge_dt_,
-                                          surge_tf_, gc_, true, true));
-        surge_series_->update(t0);
-    }
 
     const size_t N = (size_t)gc_.h_nx * gc_.h_ny;
     std::vector<double> BX((size_t)(gc_.h_nx + 1) * gc_.h_ny), BY((size_t)gc_.h_nx * (gc_.h_ny + 1));
@@ -350,6 +373,12 @@ void Simulator::InitSimulation()
                              h_init_ ? h0.data() : nullptr),
           "rift_upload_static");
     check(rift_init(grid_, t0), "rift_init");
+    // the surge series starts its window on the initialised device state
+    if (surge_grid_) {
+        surge_series_.reset(new MapSeries(grid_, RIFT_MAP_SURGE, surge_prefix_, 1.0, (int)surge_dt_,
+                                          surge_tf_, gc_, true, true));
+        surge_series_->update(t0);
+    }
     bc_.assign(N, 0.0);
     check(rift_download(grid_, RIFT_BC, bc_.data()), "download BC");
     if (b_print_) write_grid(output_dir + "/bc.txt", bc_.data(), gc_);

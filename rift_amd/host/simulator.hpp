@@ -20,8 +20,10 @@ namespace rift {
 
 // A series of forcing maps `prefix-###.txt` (GridSeries.cu:33-404).  Held
 // series (rain, drain) hold the map of the next map time on plane 0;
-// interpolated series (surge, or rain with hyetograph_interp) keep two
-// planes and interpolate in time on the device.
+// interpolated series keep two planes and interpolate in time on the
+// device: rain with hyetograph_interp through map planes 1 and 2 and one
+// interpolation launch per step, surge through the device series (the
+// device advances the window; the host loads each map into a slot).
 class MapSeries {
 public:
     MapSeries(rift_grid* g, int which, std::string prefix, double scale, int deltat, double tmax,
@@ -31,7 +33,9 @@ public:
 
 private:
     void read_map(int index);                    // p_read_grid + p_interp
+    void load_device(int index);                 // read_map into the device series
     void upload(int plane);
+    void device_update(double t);
     void base_update(double t);
     void held_update(double t);
     void interp_update(double t);
@@ -44,6 +48,7 @@ private:
     double tmax_;
     GridConfig gc_;
     bool interpolated_, allow_nodata_;
+    bool device_ = false;                        // interpolated surge: device series
     double in_time_ = -9999.0, next_time_ = 0.0, sum_cache_ = 0.0;
     bool done_ = false;
     std::vector<double> ibuf_;                   // internal grid

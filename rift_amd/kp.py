@@ -36,7 +36,7 @@ FIELDS = ["w", "hu", "hv", "depth", "bc", "n", "K", "F", "dw", "dhu", "dhv", "G"
 SER_HYETO, SER_DRAIN, SER_HYDRO = 0, 1, 2
 MAP_RAIN, MAP_DRAIN, MAP_SURGE = 0, 1, 2
 MASS_BUDGET = 1 << 13
-OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT = 1, 2, 3
+OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT, OPT_MASS_BUDGET = 1, 2, 3, 4
 NVARIANTS = 16
 KERNEL_CLASSES = ["flux1", "stage2_flux", "integrate", "boundaries", "grow", "timestep"]
 
@@ -73,6 +73,11 @@ SIGNATURES = [
     ("rift_set_map", C.c_int, [_VP, C.c_int, C.c_int, _D]),
     ("rift_interp_map", C.c_int, [_VP, C.c_int, C.c_double]),
     ("rift_copy_map", C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
+    ("rift_map_series_create", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_double]),
+    ("rift_map_series_load", C.c_int, [_VP, C.c_int, C.c_int, _D]),
+    ("rift_map_series_start", C.c_int, [_VP, C.c_int, C.c_double]),
+    ("rift_map_series_state", C.c_int, [_VP, C.c_int, _I, _I, _D, _I]),
+    ("rift_map_cells", C.c_int, [C.c_int, C.c_int, _D, C.c_double, C.c_int, C.c_double, _D]),
     ("rift_step", C.c_int, [_VP, C.c_int]),
     ("rift_update_source", C.c_int, [_VP]),
     ("rift_grow", C.c_int, [_VP]),
@@ -334,6 +339,29 @@ class Grid:
     def copy_map(self, which, dst, src):
         _check(load().rift_copy_map(self.h, which, dst, src), "rift_copy_map")
 
+    def map_series(self, which, nslots, deltat, tmax):
+        """Device-resident interpolated map series (rift_map_series_create)."""
+        _check(load().rift_map_series_create(self.h, which, int(nslots), int(deltat), float(tmax)),
+               "rift_map_series_create")
+
+    def load_map(self, which, index, cells):
+        """Map `index` (internal grid, (ny, nx) doubles) into its slot."""
+        a = np.ascontiguousarray(cells, dtype=np.float64)
+        assert a.shape == (self.ny, self.nx), a.shape
+        _check(load().rift_map_series_load(self.h, which, int(index), _dp(a)), "rift_map_series_load")
+
+    def start_maps(self, which, t):
+        """The series' first p_update at time t (after init / set_time)."""
+        _check(load().rift_map_series_start(self.h, which, float(t)), "rift_map_series_start")
+
+    def map_state(self, which):
+        """(k0, k1, factor, first missing map index or -1) of the window in force."""
+        k0, k1, miss = C.c_int(), C.c_int(), C.c_int()
+        f = C.c_double()
+        _check(load().rift_map_series_state(self.h, which, C.byref(k0), C.byref(k1), C.byref(f),
+                                            C.byref(miss)), "rift_map_series_state")
+        return k0.value, k1.value, f.value, miss.value
+
     # --- stepping ------------------------------------------------------------
     def step(self, n=1):
         _check(load().rift_step(self.h, n), "rift_step")
@@ -352,6 +380,10 @@ class Grid:
     def set_variant(self, v):
         """Launch variant of the fused step's marching kernels (0..NVARIANTS-1)."""
         _check(load().rift_set_option(self.h, OPT_VARIANT, int(v)), "rift_set_option")
+
+    def set_budget(self, on):
+        """FP64 volume budget kernels on (resets the budget) or off."""
+        _check(load().rift_set_option(self.h, OPT_MASS_BUDGET, 1 if on else 0), "rift_set_option")
 
     def autotune(self, nsteps=16):
         """Times every launch variant on a copy of the current state, selects
@@ -478,6 +510,18 @@ def grid_from_rasters(dem, dx, *, h0=None, n=None, n_const=0.0, K=None, psi=0.0,
     if per_phase:
         g.set_path(True)
     return g
+
+
+def map_cells(corners, scale=1.0, allow_nodata=True, nodata=-9999.0):
+    """GridSeries::p_interp: the internal-grid map ((b_ny+3, b_nx+3) doubles)
+    of a corner map (b_ny, b_nx) -- four-corner means where more than two
+    corners hold data, times `scale`; nodata (or 0) elsewhere."""
+    c = np.ascontiguousarray(corners, dtype=np.float64)
+    b_ny, b_nx = c.shape
+    out = np.empty((b_ny + 3, b_nx + 3))
+    _check(load().rift_map_cells(b_nx, b_ny, _dp(c), float(scale), int(bool(allow_nodata)), float(nodata),
+                                 _dp(out)), "rift_map_cells")
+    return out
 
 
 def read_series(path):

@@ -123,23 +123,51 @@ def rain_basin(n_cells=2048, seed=2):
             "hyeto_hours": hyeto_hours, "hyeto_mmhr": hyeto_mmhr, "tf": 7200.0}
 
 
-def coastal_surge(n_cells=8192, seed=3, nmaps=37):
-    """Coastal ramp with surge maps on the low (south) edge and a hydrograph."""
+SURGE_NODATA = -9999.0
+
+
+def surge_strip_rows(n_cells):
+    """Corner rows of the surge strip on the low (south) edge."""
+    return max(4, (n_cells + 1) // 64)
+
+
+def surge_depth(k, surge_dt=600):
+    """Surge depth of map k: 2.5 sin(pi t / 2 h) at t = k surge_dt, clipped >= 0."""
+    return max(0.0, 2.5 * np.sin(np.pi * (k * float(surge_dt)) / (2 * 3600.0)))
+
+
+def surge_map(n_cells, k, surge_dt=600):
+    """Corner map k of the coastal surge series: the surge depth on the strip
+    of the low edge, nodata elsewhere ((n_cells+1)^2 doubles)."""
+    N = n_cells + 1
+    m = np.full((N, N), SURGE_NODATA)
+    m[:surge_strip_rows(n_cells), :] = surge_depth(k, surge_dt)
+    return m
+
+
+def coastal_surge(n_cells=8192, seed=3, nmaps=37, with_maps=True):
+    """Coastal ramp with surge maps on the low (south) edge and a hydrograph.
+    with_maps=False leaves the maps out (`surge_map` makes map k on demand:
+    37 maps at 8192 x 8192 would be ~20 GB of doubles)."""
     N = n_cells + 1
     y = np.linspace(0.0, 1.0, N)[:, None]
     dem = -5.0 + 35.0 * y + 4.0 * (fractal_dem(n_cells, 0.6, seed) - 0.5)
     n_map = np.where(dem < 0.0, 0.02, 0.06)
-    nodata = -9999.0
-    strip = max(4, N // 64)
-    maps = np.full((nmaps, N, N), nodata)
-    for k in range(nmaps):
-        depth = max(0.0, 2.5 * np.sin(np.pi * (k * 600.0) / (2 * 3600.0)))
-        maps[k, :strip, :] = depth
     hydro_hours = np.array([0.0, 1.0, 3.0, 6.0], dtype=np.float32)
     hydro_cfs = np.array([0.0, 500.0, 500.0, 0.0], dtype=np.float32)
-    return {"dem": dem, "n": n_map, "dx": 5.0, "surge_maps": maps, "surge_dt": 600,
-            "surge_tf": 21600.0, "hydro_hours": hydro_hours, "hydro_cfs": hydro_cfs,
-            "source_rect": (N // 2 - 10, N // 2 + 10, N // 2, N // 2 + 20), "tf": 21600.0}
+    out = {"dem": dem, "n": n_map, "dx": 5.0, "surge_dt": 600,
+           "surge_tf": 21600.0, "hydro_hours": hydro_hours, "hydro_cfs": hydro_cfs,
+           "source_rect": (N // 2 - 10, N // 2 + 10, N // 2, N // 2 + 20), "tf": 21600.0}
+    if with_maps:
+        out["surge_maps"] = np.stack([surge_map(n_cells, k) for k in range(nmaps)])
+    return out
+
+
+def source_cells(nx, rect):
+    """Internal-grid cell indices j*nx + i of a source rectangle
+    (i0, i1, j0, j1), i outer and j inner (InitializeSources order)."""
+    i0, i1, j0, j1 = rect
+    return np.array([j * nx + i for i in range(i0, i1 + 1) for j in range(j0, j1 + 1)], dtype=np.int32)
 
 
 def series_hours_to_seconds(hours, values):

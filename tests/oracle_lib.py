@@ -75,6 +75,7 @@ def lib(fp32=False):
     L.orc_kappa.restype = C.c_double
     L.orc_kappa.argtypes = [C.c_double, C.c_double]
     L.orc_grow_tiles.argtypes = [C.c_int, C.c_int, _U8, _I]
+    L.orc_track_tiles.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _D, C.c_int, C.c_double, _I, _U8]
     L.orc_create.restype = C.c_void_p
     L.orc_create.argtypes = [C.POINTER(OrcParams), _D, _D, _D, _D]
     L.orc_destroy.argtypes = [C.c_void_p]
@@ -84,6 +85,8 @@ def lib(fp32=False):
     L.orc_set_series_hours.argtypes = [C.c_void_p, C.c_int, _F, _F, C.c_int]
     L.orc_set_sources.argtypes = [C.c_void_p, _I, C.c_long]
     L.orc_set_map_series.argtypes = [C.c_void_p, C.c_int, _D, C.c_int, C.c_int, C.c_double]
+    L.orc_set_map_series_range.argtypes = [C.c_void_p, C.c_int, C.c_int, _D, C.c_int, C.c_int,
+                                           C.c_double]
     for name in ("orc_step", "orc_update_source", "orc_grow", "orc_compute_fluxes",
                  "orc_compute_timestep", "orc_integrate_1", "orc_integrate_2",
                  "orc_apply_boundaries", "orc_update_maps"):
@@ -153,10 +156,12 @@ class Oracle:
         a = np.ascontiguousarray(idx, dtype=np.int32)
         self.L.orc_set_sources(self.s, a.ctypes.data_as(_I), len(a))
 
-    def set_map_series(self, which, maps, deltat, tmax):
+    def set_map_series(self, which, maps, deltat, tmax, first=0):
+        """Corner maps of file indices first, first+1, ...; the first
+        p_update runs at the oracle's current time."""
         m = np.ascontiguousarray(maps, dtype=np.float64)
-        assert self.L.orc_set_map_series(self.s, which, dptr(m), m.shape[0], int(deltat),
-                                         float(tmax)) == 0
+        assert self.L.orc_set_map_series_range(self.s, which, int(first), dptr(m), m.shape[0],
+                                               int(deltat), float(tmax)) == 0
 
     def step(self, n=1):
         self.L.orc_run(self.s, n)
@@ -235,3 +240,22 @@ class Oracle:
 
     def interior(self, a):
         return a[2:self.ny - 2, 2:self.nx - 2]
+
+
+def track_and_grow(nbx, nby, wet, act, maps, nodata=-9999.0, fp32=False):
+    """The oracle's Track (grid.cu:1224-1276) and Grow (grid.cu:1188-1322) on
+    given tile flags: wet (ntiles u8) and act (ntiles: 1 active, 0 not) of the
+    last Grow, maps = [(kind, (ny, nx) map)] with kind 0 rain, 1 surge.
+    Returns (wet after Track, active flags after Grow)."""
+    L = lib(fp32)
+    nt = nbx * nby
+    w = np.ascontiguousarray(wet, dtype=np.uint8).copy()
+    aval = np.where(np.asarray(act) > 0, np.arange(nt), -np.arange(nt)).astype(np.int32)
+    for kind, m in maps:
+        a = np.ascontiguousarray(m, dtype=np.float64)
+        ny, nx = a.shape
+        L.orc_track_tiles(nx, ny, nbx, nby, dptr(a), int(kind), float(nodata), aval.ctypes.data_as(_I),
+                          w.ctypes.data_as(_U8))
+    out = np.empty(nt, dtype=np.int32)
+    L.orc_grow_tiles(nbx, nby, w.ctypes.data_as(_U8), out.ctypes.data_as(_I))
+    return w, (out >= 0).astype(np.uint8) | (np.arange(nt) == 0)

@@ -1,7 +1,7 @@
 """bench.py's command-line contract on the CPU: more than one GPU is refused
-with a message and exit status 2 before any device work, and the measured
-HBM traffic of the 2048 x 2048 workload is read from the committed profile
-summary (no other workload has one)."""
+with a message and exit status 2 before any device work; measured HBM
+traffic is taken from a committed profile summary only when that summary
+was recorded on the same state as the run."""
 import os
 import subprocess
 import sys
@@ -20,11 +20,21 @@ def test_more_than_one_gpu_is_refused(n):
     assert r.stdout == ""
 
 
-def test_traffic_comes_from_the_committed_profile():
+def test_traffic_only_for_the_profiled_state(tmp_path):
     sys.path.insert(0, ROOT)
     import bench
-    t = bench.measured_traffic("k_march<2>", 2048)
-    assert t is not None and 2.5e8 < t < 5e8
-    # the figure exceeds the kernel's contract bytes for 2048 x 2048 wet cells
-    assert t > bench.KERNEL_BYTES_PER_CELL["march2"] * 2048 * 2048
-    assert bench.measured_traffic("k_march<2>", 1024) is None
+    p = tmp_path / "summary.md"
+    p.write_text("# summary\nstate: coastal 64 t0=10800 spin=5 active=7\n"
+                 "traffic_bytes_per_launch k_march<2>: 1234.5\n")
+    assert bench.measured_traffic(str(p), "k_march<2>", "coastal 64 t0=10800 spin=5 active=7") == 1234.5
+    # another state (other active tile count) or another kernel: no figure
+    assert bench.measured_traffic(str(p), "k_march<2>", "coastal 64 t0=10800 spin=5 active=8") is None
+    assert bench.measured_traffic(str(p), "k_march<1>", "coastal 64 t0=10800 spin=5 active=7") is None
+    assert bench.measured_traffic(str(tmp_path / "absent.md"), "k_march<2>", "x") is None
+
+
+def test_stage_bytes_add_up_to_the_step():
+    sys.path.insert(0, ROOT)
+    import bench
+    for w in bench.STAGE_BYTES.values():
+        assert w["march1"] + w["march2"] == w["step"]

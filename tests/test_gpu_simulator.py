@@ -77,6 +77,12 @@ def _lockstep(rift, cfg_path, o, dt_print=None):
         steps += 1
         ot, _ = o.time
         assert abs(t - ot) <= DT_RTOL * ot, (steps, t, ot)
+        # wet and active tile sets (Grow with TrackRainfall/TrackSurge) are
+        # integer work: identical every step
+        gw, ga, _ = sim.grid.tiles()
+        ow, oa, _ = o.tiles()
+        np.testing.assert_array_equal(gw, ow, err_msg=f"wet tiles, step {steps}")
+        assert set(np.nonzero(ga)[0].tolist()) == set(int(v) for v in oa if v > -1), steps
         if dt_print is not None and t > t_print:
             prints.append(o.interior(o.depth()).copy())
             t_print += dt_print
