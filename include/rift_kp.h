@@ -261,6 +261,16 @@ int rift_get_rates(rift_grid* g, float* out3);
 
 /* FP64 sum of a field over interior cells (SumReduce, grid.cu:264-290). */
 int rift_sum_interior(rift_grid* g, int field, double* out);
+/* rift_sum_interior without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid it).  Per-block
+ * FP64 partials and the completion ticket live in a static device array of
+ * 32.9 KB per GPU (held until the process ends), cleared by a kernel, not a
+ * memset; the partials are summed in a fixed order, so the result does not
+ * depend on completion order.  At most 8192 x 8192 interior cells (larger
+ * domains: RIFT_E_UNSUPPORTED).  Calls that overlap on one GPU share the
+ * array and race; the call blocks until the sum is on the host. */
+int rift_sum_interior_noalloc(rift_grid* g, int field, double* out);
 /* FP64 mass balance accumulated on the device since rift_init:
  * out[0] = stored water volume (m^3), out[1] = infiltrated volume,
  * out[2] = rain volume added, out[3] = source volume added,

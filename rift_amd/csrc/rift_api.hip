@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -178,6 +179,100 @@ float* map_plane(rift_grid* g, int which, int plane)
 {
     if (which < 0 || which > 2 || plane < 0 || plane > 2) return nullptr;
     return g->d.map[which][plane];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// FP64 interior sum without library allocation (rift_sum_interior_noalloc).
+// Scratch lives in one file-scope device array per GPU: per-block FP64
+// partials, the result, and the completion ticket.  Host code reaches it
+// through the address hipGetSymbolAddress returns for the current device
+// (the array's name on the host is only a stand-in).  Every call on a GPU
+// shares that GPU's array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kSumCellsPerBlock = 256 * 64;     // interior cells per workgroup
+constexpr int kSumMaxBlocks = 4096;             // partials: up to 8192^2 interior cells
+constexpr int kSumResult = kSumMaxBlocks;       // index of the result
+constexpr int kSumTicket = kSumMaxBlocks + 8;   // index of the ticket (as unsigned bits)
+constexpr int kSumScratch = kSumMaxBlocks + 16; // doubles (32.9 KB per GPU)
+
+static __device__ __align__(256) double scratch[kSumScratch];
+
+__global__ __launch_bounds__(64) void k_sum_scratch_clear(double* sc)
+{
+    if (threadIdx.x == 0) reinterpret_cast<unsigned*>(sc + kSumTicket)[0] = 0u;
+}
+
+// Each workgroup sums its kSumCellsPerBlock consecutive interior cells
+// (row-major) in FP64 into its partial; the last to finish sums the partials
+// in a fixed order (strided per lane, then a fixed tree), so the result does
+// not depend on completion order.
+__global__ __launch_bounds__(256) void k_sum_partials(Dev d, const float* x, double* sc)
+{
+    __shared__ double red[256];
+    __shared__ bool last;
+    const int ni = d.nx - 4, nj = d.ny - 4;
+    const long total = (long)ni * nj;
+    const long k0 = (long)blockIdx.x * kSumCellsPerBlock;
+    const long k1 = min(k0 + kSumCellsPerBlock, total);
+    double acc = 0.0;
+    for (long k = k0 + threadIdx.x; k < k1; k += 256) {
+        const int j = (int)(k / ni) + 2, i = (int)(k % ni) + 2;
+        acc += (double)x[cell(d, i, j)];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sc[blockIdx.x] = red[0];
+        __threadfence();
+        last = atomicAdd(reinterpret_cast<unsigned*>(sc + kSumTicket), 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double a = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) a += __hip_atomic_load(sc + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sc[kSumResult] = red[0];
+        reinterpret_cast<unsigned*>(sc + kSumTicket)[0] = 0u;
+    }
+}
+
+// Device address of `scratch` per device, resolved once with the device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for(int device, double** out)
+{
+    if (device < 0 || device >= kMaxDevices) return fail(RIFT_E_ARG, "scratch: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        HIPCHK(hipGetDevice(&cur));
+        HIPCHK(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(RIFT_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return RIFT_OK;
 }
 
 }  // namespace
@@ -1189,6 +1284,37 @@ int rift_sum_interior(rift_grid* g, int field, double* out)
     int rc = check_launch("sum_interior");
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(out, g->dsum, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return RIFT_OK;
+}
+
+int rift_sum_interior_noalloc(rift_grid* g, int field, double* out)
+{
+    if (!g || !out) return fail(RIFT_E_ARG, "rift_sum_interior_noalloc: null argument");
+    const long cells = (long)(g->d.nx - 4) * (long)(g->d.ny - 4);
+    const long blocks = (cells + kSumCellsPerBlock - 1) / kSumCellsPerBlock;
+    if (blocks > kSumMaxBlocks)
+        return fail(RIFT_E_UNSUPPORTED, "rift_sum_interior_noalloc: " + std::to_string(cells) +
+                                            " interior cells exceed the static scratch (at most " +
+                                            std::to_string((long)kSumMaxBlocks * kSumCellsPerBlock) + ")");
+    if (blocks < 1) {
+        *out = 0.0;
+        return RIFT_OK;
+    }
+    {
+        int rc0 = ensure_canonical(g);
+        if (rc0) return rc0;
+    }
+    float* p = field_plane(g, field);
+    if (!p) return fail(RIFT_E_ARG, "rift_sum_interior_noalloc: field not available");
+    double* sc = nullptr;
+    int rc = scratch_for(g->desc.device, &sc);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(g->desc.device));
+    hipLaunchKernelGGL(k_sum_scratch_clear, dim3(1), dim3(64), 0, g->stream, sc);
+    hipLaunchKernelGGL(k_sum_partials, dim3((unsigned)blocks), dim3(256), 0, g->stream, g->d, (const float*)p, sc);
+    if ((rc = check_launch("sum_interior_noalloc"))) return rc;
+    HIPCHK(hipMemcpyAsync(out, sc + kSumResult, sizeof(double), hipMemcpyDeviceToHost, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
     return RIFT_OK;
 }

@@ -100,6 +100,7 @@ SIGNATURES = [
     ("rift_get_desc", C.c_int, [_VP, C.POINTER(RiftDesc)]),
     ("rift_get_rates", C.c_int, [_VP, _F]),
     ("rift_sum_interior", C.c_int, [_VP, C.c_int, _D]),
+    ("rift_sum_interior_noalloc", C.c_int, [_VP, C.c_int, _D]),
     ("rift_mass_balance", C.c_int, [_VP, _D]),
     ("rift_budget", C.c_int, [_VP, _D]),
     ("rift_budget_reset", C.c_int, [_VP]),
@@ -454,6 +455,13 @@ class Grid:
     def sum_interior(self, name):
         out = C.c_double()
         _check(load().rift_sum_interior(self.h, FIELDS.index(name), C.byref(out)), "rift_sum_interior")
+        return out.value
+
+    def sum_interior_noalloc(self, name):
+        """rift_sum_interior through the library's static scratch (no allocation)."""
+        out = C.c_double()
+        _check(load().rift_sum_interior_noalloc(self.h, FIELDS.index(name), C.byref(out)),
+               "rift_sum_interior_noalloc")
         return out.value
 
     def budget(self):
