@@ -49,8 +49,6 @@ struct Scalars {
     // and installed by the next grow2 when next_valid is set
     int n_units_acc;
     int next_valid;
-    int n_active_acc;                 // dt kernel: cross-workgroup accumulators
-    unsigned long long wet_cells_acc;
     // volume budget of the fused step (m summed over cells): gridded rain,
     // drain, outflow through the domain boundary, dry-clamp and surge
     // exchange; b_flux1 = stage-1 boundary outflow rate (m/s summed over
@@ -71,6 +69,14 @@ struct Scalars {
     float sg_f;
     int sg_missing;
 };
+
+// One workgroup's share of the fused step's dt reduction.
+struct DtPartial {
+    float vx, vy;
+    int nact, pad;
+    unsigned long long cells;
+};
+constexpr int kDtMaxBlocks = 512;   // workgroups of the dt kernel (partials per handle)
 
 // Surge window of one step: map indices of the two planes and the factor.
 struct SurgeWindow {
@@ -99,6 +105,7 @@ struct Dev {
     float *mx, *my;              // per-tile maxima of the last flux pass
     float *mx1, *my1;            // fused step: stage-1 maxima of this step
     Scalars* sc;
+    DtPartial* dtp;              // kDtMaxBlocks partials of the fused dt kernel
     SeriesTable* ser;            // [3]: hyetograph, drain, hydrograph
     int P, nx, ny, nbx, nby, ntiles;
     float inv_dx, inv_dy, kappa, psi_dtheta, nodata;
@@ -286,7 +293,8 @@ void launch_grow2(const Dev& d, hipStream_t s);
 void launch_timestep2(const Dev& d, hipStream_t s);
 void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks);
 // workgroups of a marching kernel resident at once on a device of `cus` CUs
-int march_resident_blocks(int stage, int variant, unsigned flags, int cus);
+// (for the instantiation d's flags and surge series select)
+int march_resident_blocks(const Dev& d, int stage, int variant, int cus);
 constexpr int kMarchVariants = 4;
 // Ensembles (devs: device array of the members' Dev)
 void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s);

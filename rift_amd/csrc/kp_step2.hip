@@ -217,6 +217,15 @@ __device__ __forceinline__ size_t row_of(const Dev& d, int j)
 constexpr unsigned kLeanMask = ~(RIFT_RAIN_GRID | RIFT_DRAIN_GRID | RIFT_SURGE_GRID | RIFT_HYDROGRAPH |
                                  RIFT_SAVE_MAX | RIFT_SAVE_ARRIVAL);
 constexpr unsigned kFullMask = ~0u;
+// The coastal feature set: the lean one plus the surge series and the
+// hydrograph rectangle (a surge map is always read from the device series).
+constexpr unsigned kCoastMask = ~(RIFT_RAIN_GRID | RIFT_DRAIN_GRID | RIFT_SAVE_MAX | RIFT_SAVE_ARRIVAL);
+// Instantiations of the single-domain marching kernels.
+enum MarchKind { kKindFull = 0, kKindLean = 1, kKindCoast = 2 };
+constexpr unsigned kind_mask(int kind)
+{
+    return kind == kKindLean ? kLeanMask : (kind == kKindCoast ? kCoastMask : kFullMask);
+}
 // The same features for the ensemble kernels.  Bit 31 is no flag, so the
 // masking is identical; the distinct value keeps the one-body row loop and
 // the scalar-base addressing there (their member lookup leaves no registers
@@ -235,7 +244,7 @@ constexpr unsigned kLeanEnsMask = kLeanMask & ~(1u << 31);
 template <int Stage, unsigned FM>
 __device__ __forceinline__ unsigned lane_off(unsigned x)
 {
-    if constexpr (!(Stage == 2 && FM == kLeanMask)) asm volatile("" : "+v"(x));
+    if constexpr (!(Stage == 2 && (FM == kLeanMask || FM == kCoastMask))) asm volatile("" : "+v"(x));
     return x;
 }
 
@@ -281,7 +290,8 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const StepScalars& S, const L
         r.dhu = ld(d.dhu + rb, sc4);
         r.dhv = ld(d.dhv + rb, sc4);
         if (d.flags & FM & RIFT_SURGE_GRID) {
-            if (d.sg_series) r.sg = surge_interp(ld(S.sg0 + rb, sc4), ld(S.sg1 + rb, sc4), S.sgf, d.nodata);
+            if (FM == kCoastMask || d.sg_series)
+                r.sg = surge_interp(ld(S.sg0 + rb, sc4), ld(S.sg1 + rb, sc4), S.sgf, d.nodata);
             else r.sg = ld(d.map[RIFT_MAP_SURGE][0] + rb, sc4);
         } else {
             r.sg = 0.f;
@@ -568,7 +578,7 @@ struct Phase {
 // body lengthen live ranges: the lean kernels have the registers for it, the
 // full and ensemble kernels (more pointers and flags live) would spill.
 template <unsigned FM>
-constexpr bool phased_rows() { return FM == kLeanMask; }
+constexpr bool phased_rows() { return FM == kLeanMask || FM == kCoastMask; }
 
 // One run of a wave: output rows [ja, jb) of strip s, marched downwards
 // (increasing j) or upwards.  The window holds the output row C, the next
@@ -756,9 +766,10 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
 // Otherwise: the active units (strip-major) form a sequence of rows and wave
 // i marches the i-th equal share of at least Dev::min_rows rows, downwards,
 // continuing from one unit into the next while they are vertically adjacent.
-template <int Stage, bool Textbook, int Waves, bool Budget, bool Lean>
+template <int Stage, bool Textbook, int Waves, bool Budget, int Kind>
 __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
 {
+    constexpr unsigned FM = kind_mask(Kind);
     const int lane = threadIdx.x & 63;
     // the wavefront index is uniform across the wave: in a scalar register,
     // every unit and row index below is scalar too (SALU address math,
@@ -801,8 +812,8 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         const int ja = 2 + (int)((long long)rows * mi / m);
         const int jb = 2 + (int)((long long)rows * (mi + 1) / m);
         if (ja >= jb) return;
-        if (mi & 1) march_run<Stage, Textbook, true, Budget, Lean ? kLeanMask : kFullMask>(d, S, s, ja, jb, lane);
-        else march_run<Stage, Textbook, false, Budget, Lean ? kLeanMask : kFullMask>(d, S, s, ja, jb, lane);
+        if (mi & 1) march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane);
+        else march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
         return;
     }
 
@@ -831,7 +842,7 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         const int rlast = last - s * d.nurows;
         const long long pe = min(pend, (long long)(kk + 1) * kTileH);
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (long long)kk * kTileH), d.ny - 2);
-        if (ja < jb) march_run<Stage, Textbook, false, Budget, Lean ? kLeanMask : kFullMask>(d, S, s, ja, jb, lane);
+        if (ja < jb) march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
         k = kk + 1;
         a = 0;
         if ((long long)k * kTileH >= pend) break;
@@ -875,104 +886,130 @@ __global__ __launch_bounds__(256) void k_clear_sel(Dev d)
 // dt of the fused step: max over every tile of the stage-1 maxima (active
 // tiles) or the stage-2 maxima of the last step a tile was active (others),
 // then ComputeTimestep's arithmetic.  The same pass counts the active tiles
-// and their interior cells.  Workgroups reduce grid-stride slices (all loads
-// independent, so they overlap) into device accumulators; the last workgroup
-// to finish forms dt, moves the unit count to stage 2, clears the
-// accumulators for the next step and evaluates the next step's forcing at
-// the new time (UpdateSource runs at the start of the next RunSimulation,
-// simulator.cu:780-784, with the t of this step).
-constexpr int kDtBlocks = 32;
+// and their interior cells.  Each workgroup reduces kDtTilesPerBlock
+// consecutive tiles (four independent loads per lane in flight) into its
+// slot of the handle's partials array; the last workgroup to finish (ticket)
+// reduces the partials in a fixed order, forms dt, moves the unit count to
+// stage 2 and evaluates the next step's forcing at the new time
+// (UpdateSource runs at the start of the next RunSimulation,
+// simulator.cu:780-784, with the t of this step).  No same-address atomics
+// beyond the ticket: with one atomic per workgroup and quantity, the 300k
+// tiles of an 8192^2 domain took 41 us over 32 workgroups.
+constexpr int kDtTilesPerBlock = 1024;
+
+__host__ __device__ inline int dt_blocks(int ntiles)
+{
+    const int b = (ntiles + kDtTilesPerBlock - 1) / kDtTilesPerBlock;
+    return b < 1 ? 1 : (b > kDtMaxBlocks ? kDtMaxBlocks : b);
+}
 
 __device__ __forceinline__ void timestep2_body(const Dev& d)
 {
-    __shared__ float red[2][4];
-    __shared__ unsigned long long redc[4];
-    __shared__ int redn[4];
+    __shared__ float redx[256], redy[256];
+    __shared__ unsigned long long redc[256];
+    __shared__ int redn[256];
     __shared__ bool last;
     float vx = 0.f, vy = 0.f;
     unsigned long long cells = 0ull;
     int nact = 0;
-    for (int t = blockIdx.x * 256 + threadIdx.x; t < d.ntiles; t += gridDim.x * 256) {
-        const bool a = d.act[t] != 0;
-        const float x1 = d.mx1[t], x0 = d.mx[t], y1 = d.my1[t], y0 = d.my[t];
-        vx = fmaxf(vx, a ? x1 : x0);
-        vy = fmaxf(vy, a ? y1 : y0);
-        if (a) {
-            const int bx = t % d.nbx, by = t / d.nbx;
-            const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
-            const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
-            if (i1 > i0 && j1 > j0) cells += (unsigned long long)((i1 - i0) * (j1 - j0));
-            ++nact;
+    const int tid = threadIdx.x;
+    // consecutive tiles per workgroup; a grid capped at kDtMaxBlocks strides
+    for (int base = blockIdx.x * kDtTilesPerBlock; base < d.ntiles; base += gridDim.x * kDtTilesPerBlock) {
+#pragma unroll
+        for (int k = 0; k < kDtTilesPerBlock / 256; ++k) {
+            const int t = base + k * 256 + tid;
+            if (t >= d.ntiles) break;
+            const bool a = d.act[t] != 0;
+            const float x1 = d.mx1[t], x0 = d.mx[t], y1 = d.my1[t], y0 = d.my[t];
+            vx = fmaxf(vx, a ? x1 : x0);
+            vy = fmaxf(vy, a ? y1 : y0);
+            if (a) {
+                const int bx = t % d.nbx, by = t / d.nbx;
+                const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
+                const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
+                if (i1 > i0 && j1 > j0) cells += (unsigned long long)((i1 - i0) * (j1 - j0));
+                ++nact;
+            }
         }
     }
-    vx = wave_max(vx);
-    vy = wave_max(vy);
-    for (int off = 32; off > 0; off >>= 1) {
-        cells += __shfl_xor(cells, off);
-        nact += __shfl_xor(nact, off);
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[0][w] = vx;
-        red[1][w] = vy;
-        redc[w] = cells;
-        redn[w] = nact;
-    }
+    // fixed-order tree over the workgroup
+    redx[tid] = vx;
+    redy[tid] = vy;
+    redc[tid] = cells;
+    redn[tid] = nact;
     __syncthreads();
-    Scalars* sc = d.sc;
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < 4; ++k) {
-            vx = fmaxf(vx, red[0][k]);
-            vy = fmaxf(vy, red[1][k]);
-            cells += redc[k];
-            nact += redn[k];
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            redx[tid] = fmaxf(redx[tid], redx[tid + w]);
+            redy[tid] = fmaxf(redy[tid], redy[tid + w]);
+            redc[tid] += redc[tid + w];
+            redn[tid] += redn[tid + w];
         }
-        atomicMax(&sc->mx_bits, __float_as_uint(vx));   // speeds are >= 0
-        atomicMax(&sc->my_bits, __float_as_uint(vy));
-        atomicAdd(&sc->wet_cells_acc, cells);
-        atomicAdd(&sc->n_active_acc, nact);
+        __syncthreads();
+    }
+    Scalars* sc = d.sc;
+    if (tid == 0) {
+        DtPartial* p = d.dtp + blockIdx.x;
+        __hip_atomic_store(&p->vx, redx[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->vy, redy[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->cells, redc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->nact, redn[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         last = atomicAdd(&sc->ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0 && sc->finished) {
-        // an ended domain keeps its time; the accumulators are cleared
-        sc->mx_bits = 0u;
-        sc->my_bits = 0u;
-        sc->wet_cells_acc = 0ull;
-        sc->n_active_acc = 0;
-        sc->ticket = 0u;
+    if (!last) return;
+    __threadfence();
+    vx = vy = 0.f;
+    cells = 0ull;
+    nact = 0;
+    for (int b = tid; b < (int)gridDim.x; b += 256) {
+        const DtPartial* p = d.dtp + b;
+        vx = fmaxf(vx, __hip_atomic_load(&p->vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        vy = fmaxf(vy, __hip_atomic_load(&p->vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        cells += __hip_atomic_load(&p->cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        nact += __hip_atomic_load(&p->nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    redx[tid] = vx;
+    redy[tid] = vy;
+    redc[tid] = cells;
+    redn[tid] = nact;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            redx[tid] = fmaxf(redx[tid], redx[tid + w]);
+            redy[tid] = fmaxf(redy[tid], redy[tid + w]);
+            redc[tid] += redc[tid + w];
+            redn[tid] += redn[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    sc->ticket = 0u;
+    if (sc->finished) {
+        // an ended domain keeps its time; the step's counters are cleared
         sc->n_units = 0;
         sc->n_units_acc = 0;
         sc->next_valid = 0;
         sc->b_flux1 = 0.0;
-    } else if (last && threadIdx.x == 0) {
-        __threadfence();
-        const float vx_all = __uint_as_float(atomicAdd(&sc->mx_bits, 0u));
-        const float vy_all = __uint_as_float(atomicAdd(&sc->my_bits, 0u));
-        const unsigned long long cells_all = atomicAdd(&sc->wet_cells_acc, 0ull);
-        sc->n_active = atomicAdd(&sc->n_active_acc, 0);
-        sc->wet_cells = cells_all;
-        sc->wcu_total += cells_all;
-        finish_timestep(d, vx_all, vy_all);
-        if (d.flags & RIFT_MASS_BUDGET) {
-            sc->b_out += 0.5 * sc->dt * sc->b_flux1;
-            sc->b_flux1 = 0.0;
-        }
-        sc->mx_bits = 0u;
-        sc->my_bits = 0u;
-        sc->wet_cells_acc = 0ull;
-        sc->n_active_acc = 0;
-        sc->ticket = 0u;
-        sc->n_units = sc->n_units_acc;
-        sc->n_units_acc = 0;
-        const Forcing f = forcing_at(d, sc);
-        sc->nxt_rain = f.rain;
-        sc->nxt_drain = f.drain;
-        sc->nxt_hydro = f.hydro;
-        sc->nxt_cell = f.cell;
-        sc->next_valid = 1;
+        return;
     }
+    sc->n_active = redn[0];
+    sc->wet_cells = redc[0];
+    sc->wcu_total += redc[0];
+    finish_timestep(d, redx[0], redy[0]);
+    if (d.flags & RIFT_MASS_BUDGET) {
+        sc->b_out += 0.5 * sc->dt * sc->b_flux1;
+        sc->b_flux1 = 0.0;
+    }
+    sc->n_units = sc->n_units_acc;
+    sc->n_units_acc = 0;
+    const Forcing f = forcing_at(d, sc);
+    sc->nxt_rain = f.rain;
+    sc->nxt_drain = f.drain;
+    sc->nxt_hydro = f.hydro;
+    sc->nxt_cell = f.cell;
+    sc->next_valid = 1;
 }
 
 __global__ __launch_bounds__(256) void k_timestep2(Dev d)
@@ -982,8 +1019,7 @@ __global__ __launch_bounds__(256) void k_timestep2(Dev d)
 
 void launch_timestep2(const Dev& d, hipStream_t s)
 {
-    const int blocks = std::max(1, std::min(kDtBlocks, (d.ntiles + 255) / 256));
-    hipLaunchKernelGGL(k_timestep2, dim3(blocks), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_timestep2, dim3(dt_blocks(d.ntiles)), dim3(256), 0, s, d);
 }
 
 void launch_grow2(const Dev& d, hipStream_t s)
@@ -1000,65 +1036,60 @@ void launch_grow2(const Dev& d, hipStream_t s)
 // are Dev::band and Dev::min_rows.  The grid is
 // the number of workgroups resident at once (march_resident_blocks).
 // The lean instantiation serves the non-textbook domains that use none of
-// the features outside kLeanMask.
+// the features outside kLeanMask, the coastal one those within kCoastMask
+// whose surge (if any) comes from the device series.
 bool march_lean(unsigned flags)
 {
     return !(flags & RIFT_TEXTBOOK) && (flags & ~kLeanMask) == 0u;
 }
 
-template <int Stage, int Waves>
-static void launch_march_w(const Dev& d, int blocks, hipStream_t s)
+int march_kind(const Dev& d)
 {
-    const bool tb = d.flags & RIFT_TEXTBOOK, bud = d.flags & RIFT_MASS_BUDGET;
-    if (march_lean(d.flags)) {
-        if (bud) hipLaunchKernelGGL((k_march<Stage, false, Waves, true, true>), dim3(blocks), dim3(256), 0, s, d);
-        else hipLaunchKernelGGL((k_march<Stage, false, Waves, false, true>), dim3(blocks), dim3(256), 0, s, d);
-    } else if (tb && bud) {
-        hipLaunchKernelGGL((k_march<Stage, true, Waves, true, false>), dim3(blocks), dim3(256), 0, s, d);
-    } else if (tb) {
-        hipLaunchKernelGGL((k_march<Stage, true, Waves, false, false>), dim3(blocks), dim3(256), 0, s, d);
-    } else if (bud) {
-        hipLaunchKernelGGL((k_march<Stage, false, Waves, true, false>), dim3(blocks), dim3(256), 0, s, d);
-    } else {
-        hipLaunchKernelGGL((k_march<Stage, false, Waves, false, false>), dim3(blocks), dim3(256), 0, s, d);
-    }
+    if (march_lean(d.flags)) return kKindLean;
+    if (!(d.flags & RIFT_TEXTBOOK) && (d.flags & ~kCoastMask) == 0u &&
+        (!(d.flags & RIFT_SURGE_GRID) || d.sg_series))
+        return kKindCoast;
+    return kKindFull;
+}
+
+template <int Stage, int Waves>
+static const void* march_fn(int kind, bool tb, bool bud)
+{
+    if (kind == kKindLean)
+        return bud ? (const void*)k_march<Stage, false, Waves, true, kKindLean>
+                   : (const void*)k_march<Stage, false, Waves, false, kKindLean>;
+    if (kind == kKindCoast)
+        return bud ? (const void*)k_march<Stage, false, Waves, true, kKindCoast>
+                   : (const void*)k_march<Stage, false, Waves, false, kKindCoast>;
+    return tb ? (bud ? (const void*)k_march<Stage, true, Waves, true, kKindFull>
+                     : (const void*)k_march<Stage, true, Waves, false, kKindFull>)
+              : (bud ? (const void*)k_march<Stage, false, Waves, true, kKindFull>
+                     : (const void*)k_march<Stage, false, Waves, false, kKindFull>);
+}
+
+static const void* march_kernel(int stage, int variant, int kind, unsigned flags)
+{
+    const bool tb = flags & RIFT_TEXTBOOK, bud = flags & RIFT_MASS_BUDGET;
+    if (stage == 1) return (variant & 1) ? march_fn<1, 5>(kind, tb, bud) : march_fn<1, 4>(kind, tb, bud);
+    return (variant & 2) ? march_fn<2, 4>(kind, tb, bud) : march_fn<2, 3>(kind, tb, bud);
 }
 
 void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks)
 {
     blocks = std::max(1, blocks);
-    if (stage == 1) {
-        if (variant & 1) launch_march_w<1, 5>(d, blocks, s);
-        else launch_march_w<1, 4>(d, blocks, s);
-    } else {
-        if (variant & 2) launch_march_w<2, 4>(d, blocks, s);
-        else launch_march_w<2, 3>(d, blocks, s);
-    }
+    Dev arg = d;
+    void* args[] = {&arg};
+    (void)hipLaunchKernel(march_kernel(stage, variant, march_kind(d), d.flags), dim3(blocks), dim3(256), args,
+                          0, s);
 }
 
-template <int Stage, int Waves>
-static int resident(unsigned flags, int cus)
+int march_resident_blocks(const Dev& d, int stage, int variant, int cus)
 {
-    const bool textbook = flags & RIFT_TEXTBOOK, budget = flags & RIFT_MASS_BUDGET;
-    const void* k;
-    if (march_lean(flags))
-        k = budget ? (const void*)k_march<Stage, false, Waves, true, true>
-                   : (const void*)k_march<Stage, false, Waves, false, true>;
-    else
-        k = textbook ? (budget ? (const void*)k_march<Stage, true, Waves, true, false>
-                               : (const void*)k_march<Stage, true, Waves, false, false>)
-                     : (budget ? (const void*)k_march<Stage, false, Waves, true, false>
-                               : (const void*)k_march<Stage, false, Waves, false, false>);
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, march_kernel(stage, variant, march_kind(d), d.flags), 256, 0);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     return per_cu * cus;
-}
-
-int march_resident_blocks(int stage, int variant, unsigned flags, int cus)
-{
-    if (stage == 1) return (variant & 1) ? resident<1, 5>(flags, cus) : resident<1, 4>(flags, cus);
-    return (variant & 2) ? resident<2, 4>(flags, cus) : resident<2, 3>(flags, cus);
 }
 
 void launch_canonicalize(const Dev& d, hipStream_t s)
@@ -1282,8 +1313,7 @@ void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s)
 
 void launch_timestep2_ens(const Dev* devs, int nmem, int max_tiles, hipStream_t s)
 {
-    const int blocks = std::max(1, std::min(kDtBlocks, (max_tiles + 255) / 256));
-    hipLaunchKernelGGL(k_timestep2_ens, dim3(blocks, nmem), dim3(256), 0, s, devs);
+    hipLaunchKernelGGL(k_timestep2_ens, dim3(dt_blocks(max_tiles), nmem), dim3(256), 0, s, devs);
 }
 
 template <int Stage, int Waves, bool Lean>

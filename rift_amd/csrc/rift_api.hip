@@ -381,9 +381,10 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.flip = alloc_bytes<unsigned char>(g, d.ntiles);
     d.ulist = alloc_bytes<int>(g, d.nunits);
     d.sc = alloc_bytes<Scalars>(g, 1);
+    d.dtp = alloc_bytes<DtPartial>(g, kDtMaxBlocks);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
-    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.ser && g->dsum && d.sel &&
+    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.dtp && d.ser && g->dsum && d.sel &&
          d.flip && d.ulist;
     if (!ok) {
         rift_destroy(g);
@@ -396,7 +397,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
         g->cus = prop.multiProcessorCount;
         for (int st = 0; st < 2; ++st)
             for (int v = 0; v < kMarchVariants; ++v)
-                g->march_blocks[st][v] = march_resident_blocks(st + 1, v, desc->flags, g->cus);
+                g->march_blocks[st][v] = march_resident_blocks(g->d, st + 1, v, g->cus);
     }
     *out = g;
     return RIFT_OK;
@@ -833,7 +834,7 @@ int rift_set_option(rift_grid* g, int option, int value)
         g->desc.flags = g->d.flags;
         for (int st = 0; st < 2; ++st)
             for (int v = 0; v < kMarchVariants; ++v)
-                g->march_blocks[st][v] = march_resident_blocks(st + 1, v, g->d.flags, g->cus);
+                g->march_blocks[st][v] = march_resident_blocks(g->d, st + 1, v, g->cus);
         if (on && g->inited) {
             int rc = rift_budget_reset(g);
             if (rc) return rc;
@@ -963,6 +964,9 @@ int rift_map_series_create(rift_grid* g, int which, int nslots, int deltat, doub
     d.sg_tmax = tmax;
     d.sg_series = 1;
     g->sg_dt_int = deltat;
+    // the surge series may select another instantiation of the marching kernels
+    for (int st = 0; st < 2; ++st)
+        for (int v = 0; v < kMarchVariants; ++v) g->march_blocks[st][v] = march_resident_blocks(d, st + 1, v, g->cus);
     g->primed = false;
     drop_graphs(g);
     return RIFT_OK;
