@@ -437,6 +437,8 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 def coastal_leg(args, regions):
     from rift_amd import kp
     g, cfg = build_coastal(args.cells, budget=True)
+    if not args.graphs:
+        g.set_graphs(False)
     spin_steps = spin_up(g, SPINUP_S, label="coastal spin-up")
     b = g.budget()
     g.set_budget(False)
@@ -516,6 +518,9 @@ def main():
     ap.add_argument("--rain-basin", type=int, default=1, help="run the 2048^2 full-activity leg (0: skip)")
     ap.add_argument("--ensemble", type=int, default=64,
                     help="scenarios of the batched-ensemble leg (0: skip)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="1: steps replayed from captured HIP graphs (default); 0: every kernel "
+                         "launched on its own (same kernels and results)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="launch variant of the marching kernels (default: autotuned after the "
                          "spin-up, outside the timed region)")

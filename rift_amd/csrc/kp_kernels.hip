@@ -383,7 +383,7 @@ __global__ __launch_bounds__(256) void k_timestep(Dev d)
         __threadfence();
         const float vx_all = __uint_as_float(atomicAdd(&sc->mx_bits, 0u));
         const float vy_all = __uint_as_float(atomicAdd(&sc->my_bits, 0u));
-        finish_timestep(d, vx_all, vy_all);
+        finish_timestep(d, vx_all, vy_all, sc);
         sc->mx_bits = 0u;
         sc->my_bits = 0u;
         sc->ticket = 0u;

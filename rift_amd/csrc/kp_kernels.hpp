@@ -68,6 +68,9 @@ struct Scalars {
     int sg_k0, sg_k1;
     float sg_f;
     int sg_missing;
+    // the per-unit surge flags must be recomputed by the coming step
+    // (window moved, or a map was loaded since the last dt kernel)
+    int sg_udirty, sg_udirty_load;
 };
 
 // One workgroup's share of the fused step's dt reduction.
@@ -95,6 +98,7 @@ struct Dev {
     // h/hu/hv when 0, in h0/hu0/hv0 when 1), pending flips, the active units
     unsigned char *sel, *flip;
     int* ulist;
+    unsigned char* umark;        // per unit: covers an active tile (set by grow, cleared by compaction)
     int nstrips, nurows, nunits;
     long long soff;              // h0 - h (elements): offset of the second buffer
     int n_uniform;
@@ -121,6 +125,7 @@ struct Dev {
     float* sgs;
     unsigned char* sgflag;
     int* sgtag;
+    unsigned char* sgu;          // per marching unit: surge > 0 possible this step
     int sg_series;               // 1: surge from the device series
     int sg_nslots;
     double sg_dt, sg_tmax;       // map interval (integer seconds) and surge_tf
@@ -163,6 +168,7 @@ __device__ __forceinline__ float surge_interp(float x0, float x1, float f, float
 __device__ inline void advance_surge_window(const Dev& d, Scalars* sc)
 {
     const double t = sc->t;
+    const int k0_old = sc->sg_nk0, k1_old = sc->sg_nk1;
     if (t >= d.sg_tmax) {
         if (!sc->sg_done) {
             sc->sg_in = sc->sg_next;
@@ -176,6 +182,8 @@ __device__ inline void advance_surge_window(const Dev& d, Scalars* sc)
         sc->sg_next = sc->sg_in + d.sg_dt;
         sc->sg_nk1 = (int)trunc(sc->sg_next / d.sg_dt);
     }
+    sc->sg_udirty = (sc->sg_nk0 != k0_old || sc->sg_nk1 != k1_old || sc->sg_udirty_load) ? 1 : 0;
+    sc->sg_udirty_load = 0;
     const double factor = (t - sc->sg_in) / (sc->sg_next - sc->sg_in);
     sc->sg_nf = (float)factor;
     sc->sg_next_valid = 1;
@@ -199,18 +207,24 @@ __host__ __device__ inline size_t cell(const Dev& d, int i, int j)
 // Source::InterpolateRate (source.cc:65-73) in float: inside [t_i, t_i+1) the
 // linear interpolant; outside the table the previous rate is kept (and the
 // caller divides it by the unit again, as the reference does).
-__device__ inline float series_interp(const SeriesTable& s, float tf, float rate)
+// The interval Source::InterpolateRate ends on: the last i with
+// time[i] <= tf < time[i+1] (each match overwrites the rate, so the last one
+// decides), or -1; and the rate of that interval (the previous rate if none).
+__device__ inline int series_last_match(const SeriesTable& s, float tf)
 {
-    for (int i = 0; i < s.n - 1; ++i) {
-        if (tf >= s.time[i] && tf < s.time[i + 1]) {
-            float num = __fsub_rn(s.rate[i + 1], s.rate[i]);
-            float den = __fsub_rn(s.time[i + 1], s.time[i]);
-            float q = __fdiv_rn(num, den);
-            float dd = __fsub_rn(tf, s.time[i]);
-            rate = __fadd_rn(s.rate[i], __fmul_rn(q, dd));
-        }
-    }
-    return rate;
+    int m = -1;
+    for (int i = 0; i < s.n - 1; ++i)
+        if (tf >= s.time[i] && tf < s.time[i + 1]) m = i;
+    return m;
+}
+__device__ inline float series_rate_at(const SeriesTable& s, int i, float tf, float rate)
+{
+    if (i < 0) return rate;
+    float num = __fsub_rn(s.rate[i + 1], s.rate[i]);
+    float den = __fsub_rn(s.time[i + 1], s.time[i]);
+    float q = __fdiv_rn(num, den);
+    float dd = __fsub_rn(tf, s.time[i]);
+    return __fadd_rn(s.rate[i], __fmul_rn(q, dd));
 }
 
 struct Forcing {
@@ -218,18 +232,23 @@ struct Forcing {
 };
 
 // UpdateSource + setSourceValue (simulator.cu:500-538, 891-901) at the time
-// held in sc->t, from the rates of the previous step.
-__device__ inline Forcing forcing_at(const Dev& d, const Scalars* sc)
+// held in sc->t, from the rates of the previous step.  match[k] (optional):
+// series_last_match of series k at (float)t, found beforehand.
+__device__ inline Forcing forcing_at(const Dev& d, const Scalars* sc, const int* match = nullptr)
 {
     Forcing r{sc->rain_rate, sc->drain_rate, sc->hydro_rate, sc->src_cell_rate};
     const unsigned f = d.flags;
     const float tf = (float)sc->t;
+    auto interp = [&](int k, float rate) {
+        const SeriesTable& s = d.ser[k];
+        return series_rate_at(s, match ? match[k] : series_last_match(s, tf), tf, rate);
+    };
     if (f & RIFT_HYDROGRAPH) {
         double value;
         if (sc->hydro_external) {
             value = sc->hydro_value;
         } else {
-            float q = series_interp(d.ser[RIFT_SER_HYDROGRAPH], tf, r.hydro);
+            float q = interp(RIFT_SER_HYDROGRAPH, r.hydro);
             q = __fdiv_rn(q, 35.3147f);               // cfs -> m^3/s
             r.hydro = q;
             value = (double)q;
@@ -238,11 +257,11 @@ __device__ inline Forcing forcing_at(const Dev& d, const Scalars* sc)
         r.cell = (float)per_cell;
     }
     if (f & RIFT_RAIN_AVG) {
-        const float q = series_interp(d.ser[RIFT_SER_HYETOGRAPH], tf, r.rain);
+        const float q = interp(RIFT_SER_HYETOGRAPH, r.rain);
         r.rain = __fdiv_rn(q, 3600.f * 1000.f);    // mm/hr -> m/s
     }
     if (f & RIFT_DRAIN_AVG) {
-        const float q = series_interp(d.ser[RIFT_SER_DRAIN], tf, r.drain);
+        const float q = interp(RIFT_SER_DRAIN, r.drain);
         r.drain = __fdiv_rn(q, 3600.f * 1000.f);
     }
     return r;
@@ -250,9 +269,8 @@ __device__ inline Forcing forcing_at(const Dev& d, const Scalars* sc)
 
 // ComputeTimestep's arithmetic (simulator.cu:549-561) on one lane; also the
 // FP64 accounting of the scalar inflows of the step.
-__device__ inline void finish_timestep(const Dev& d, float vx, float vy)
+__device__ inline void finish_timestep(const Dev& d, float vx, float vy, Scalars* sc)
 {
-    Scalars* sc = d.sc;
     const double max_x = (double)vx, max_y = (double)vy;
     const double c_x = d.dx / (double)fmaxf((float)(max_x / d.courant), (float)d.kappa_d);
     const double c_y = d.dy / (double)fmaxf((float)(max_y / d.courant), (float)d.kappa_d);
@@ -297,7 +315,7 @@ void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int block
 int march_resident_blocks(const Dev& d, int stage, int variant, int cus);
 constexpr int kMarchVariants = 4;
 // Ensembles (devs: device array of the members' Dev)
-void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s);
+void launch_grow2_ens(const Dev* devs, int nmem, int max_tiles, int max_units, hipStream_t s);
 void launch_timestep2_ens(const Dev* devs, int nmem, int max_tiles, hipStream_t s);
 // lean: every member runs the lean instantiation (march_lean of its flags)
 void launch_march_ens(const Dev* devs, int nmem, unsigned flags, bool lean, int stage, int min_rows,

@@ -69,11 +69,25 @@ __device__ __forceinline__ void strip_tiles(const Dev& d, int s, int& txa, int& 
     txb = (c1 - 2) / kTileW;
 }
 
+// tile columns under all 64 lanes of strip s (halo lanes included), the
+// columns clamped to the interior as the lanes clamp their loads
+__device__ __forceinline__ void strip_lane_tiles(const Dev& d, int s, int& txa, int& txb)
+{
+    const int c0 = max(kStrip * s, 2);
+    const int c1 = min(kStrip * s + 63, d.nx - 3);
+    txa = (c0 - 2) / kTileW;
+    txb = (c1 - 2) / kTileW;
+}
+
 // ---------------------------------------------------------------------------
-// grow2: threads [0, ntiles) handle tiles, [ntiles, ntiles+nunits) units.
-// Only the units are compacted (one atomic per wavefront): the marching
-// kernels walk units, and the dt kernel counts the active tiles and cells.
-// Thread 0 installs the forcing the last dt kernel evaluated for this step.
+// grow2, in two launches.  Tiles (one thread each): pending buffer flips,
+// dilation (Grow_k), reset of an active tile's maxima, and a mark on the
+// marching units whose output columns cover the tile.  Units (one thread
+// each, after every tile is marked): the unit is active when marked;
+// compacted by wavefront ballot + popcount prefix + one atomic per wave, and
+// the mark cleared for the next step.  The marching kernels walk units, the
+// dt kernel counts the active tiles and cells.  Thread 0 of the tile launch
+// installs the forcing and the surge window the last dt kernel evaluated.
 // ---------------------------------------------------------------------------
 // A domain with an end time (ensemble members) stops stepping once its time
 // has reached it: no units, no dt.
@@ -82,12 +96,11 @@ __device__ __forceinline__ bool finished(const Dev& d)
     return d.tf > 0.0 && d.sc->t >= d.tf;
 }
 
-__device__ __forceinline__ void grow2_body(const Dev& d, int t)
+__device__ __forceinline__ void grow_tiles_body(const Dev& d, int t)
 {
-    const bool done = finished(d);
     if (t == 0) {
         Scalars* sc = d.sc;
-        sc->finished = done ? 1 : 0;
+        sc->finished = finished(d) ? 1 : 0;
         if (sc->next_valid) {
             sc->rain_rate = sc->nxt_rain;
             sc->drain_rate = sc->nxt_drain;
@@ -104,45 +117,77 @@ __device__ __forceinline__ void grow2_body(const Dev& d, int t)
             if (d.sg_series) check_surge_slots(d, sc);
         }
     }
-    const bool is_tile = t < d.ntiles;
-    const bool is_unit = t >= d.ntiles && t < d.ntiles + d.nunits;
-    bool ua = false;
-    if (is_tile) {
-        if (d.flip[t]) {   // stage 2 of the last step wrote the other buffer
-            d.sel[t] ^= 1;
-            d.flip[t] = 0;
-        }
-        const bool a = dilated(d, t);
-        d.act[t] = a ? 1 : 0;
-        if (a) {
-            d.mx[t] = 0.f;
-            d.my[t] = 0.f;
-            d.mx1[t] = 0.f;
-            d.my1[t] = 0.f;
-        }
+    if (t >= d.ntiles) return;
+    if (d.flip[t]) {   // stage 2 of the last step wrote the other buffer
+        d.sel[t] ^= 1;
+        d.flip[t] = 0;
     }
-    if (is_unit && !done) {
-        // units are numbered strip-major: u = strip * nurows + tile row
-        const int u = t - d.ntiles;
-        const int s = u / d.nurows, r = u - s * d.nurows;
+    const bool a = dilated(d, t);
+    d.act[t] = a ? 1 : 0;
+    if (!a) return;
+    d.mx[t] = 0.f;
+    d.my[t] = 0.f;
+    d.mx1[t] = 0.f;
+    d.my1[t] = 0.f;
+    // units (s, by) whose output columns [60s+2, 60s+61] overlap the tile's
+    // columns [16bx+2, 16bx+17]: s from 16bx/60 to (16bx+15)/60, each kept
+    // only when the tile lies in that strip's tile range (strip_tiles)
+    const int bx = t % d.nbx, by = t / d.nbx;
+    if (by >= d.nurows) return;
+    const int s0 = (kTileW * bx) / kStrip, s1 = min((kTileW * bx + kTileW - 1) / kStrip, d.nstrips - 1);
+    for (int s = s0; s <= s1; ++s) {
         int txa, txb;
         strip_tiles(d, s, txa, txb);
-        for (int tx = txa; tx <= txb; ++tx) ua |= dilated(d, r * d.nbx + tx);
+        if (bx >= txa && bx <= txb) d.umark[s * d.nurows + by] = 1;
     }
-    // waves that hold tiles only skip the ballot (uniform per wavefront)
-    if (__ballot(is_unit) == 0ull) return;
+}
+
+// Surge relevance of every unit for the window in force: can it override a
+// cell under any of the strip's 64 lanes (halo lanes included) in the
+// unit's tile row?  Recomputed only when the window's maps or a slot's
+// contents changed (sg_udirty, set by the dt kernel and the map loads).
+__device__ __forceinline__ void unit_surge_flags(const Dev& d, int u, int s, int r)
+{
+    const Scalars* sc = d.sc;
+    const unsigned char* f0 = d.sgflag + (size_t)sg_slot(d, sc->sg_nk0) * d.ntiles;
+    const unsigned char* f1 = d.sgflag + (size_t)sg_slot(d, sc->sg_nk1) * d.ntiles;
+    int la, lb;
+    strip_lane_tiles(d, s, la, lb);
+    bool rel = false;
+    for (int tx = la; tx <= lb; ++tx) rel |= ((f0[r * d.nbx + tx] | f1[r * d.nbx + tx]) & 2) != 0;
+    d.sgu[u] = rel ? 1 : 0;
+}
+
+__device__ __forceinline__ void compact_units_body(const Dev& d, int u)
+{
+    const bool is_unit = u < d.nunits;
+    bool ua = false;
+    if (is_unit) {
+        ua = d.umark[u] != 0;
+        if (ua) d.umark[u] = 0;
+        if (finished(d)) ua = false;
+        if ((d.flags & RIFT_SURGE_GRID) && d.sg_series && d.sc->sg_udirty) {
+            const int s = u / d.nurows;
+            unit_surge_flags(d, u, s, u - s * d.nurows);
+        }
+    }
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
     const unsigned long long mu = __ballot(ua);
     int ubase = 0;
     if (lane == 0 && mu) ubase = atomicAdd(&d.sc->n_units_acc, __popcll(mu));
     ubase = __shfl(ubase, 0);
-    if (ua) d.ulist[ubase + __popcll(mu & lt)] = t - d.ntiles;
+    if (ua) d.ulist[ubase + __popcll(mu & lt)] = u;
 }
 
-__global__ __launch_bounds__(256) void k_grow2(Dev d)
+__global__ __launch_bounds__(256) void k_grow_tiles(Dev d)
 {
-    grow2_body(d, blockIdx.x * blockDim.x + threadIdx.x);
+    grow_tiles_body(d, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_compact_units(Dev d)
+{
+    compact_units_body(d, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,6 +227,9 @@ struct Lane {
     bool sel0, sel1, sel2;   // buffer selects of tile rows r-1, r, r+1
     bool act0, act1, act2;   // activity of tile rows r-1, r, r+1
     bool selN, actN;         // the same for tile row r+2 (down) or r-2 (up)
+    // surge series: the strip's tiles of tile rows r-1, r, r+1 (and the next
+    // one) can hold surge > 0 in this step's window (wave-uniform)
+    bool sg0, sg1, sg2, sgN;
 };
 
 __device__ __forceinline__ float ld(const float* row, unsigned byte_off)
@@ -289,7 +337,9 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const StepScalars& S, const L
         r.dh = ld(d.dh + rb, sc4);
         r.dhu = ld(d.dhu + rb, sc4);
         r.dhv = ld(d.dhv + rb, sc4);
-        if (d.flags & FM & RIFT_SURGE_GRID) {
+        // surge only where this step's window can hold surge > 0 in the
+        // strip's tiles of the row's tile row (a uniform branch)
+        if ((d.flags & FM & RIFT_SURGE_GRID) && pick(slot, L.sg0, L.sg1, L.sg2)) {
             if (FM == kCoastMask || d.sg_series)
                 r.sg = surge_interp(ld(S.sg0 + rb, sc4), ld(S.sg1 + rb, sc4), S.sgf, d.nodata);
             else r.sg = ld(d.map[RIFT_MAP_SURGE][0] + rb, sc4);
@@ -358,7 +408,21 @@ __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& ra
 }
 
 // Lane constants for strip s, tile row rr.
-template <bool Up>
+// Surge relevance of unit (strip s, tile row r), clamped to the tile rows.
+template <unsigned FM>
+__device__ __forceinline__ bool unit_surge(const Dev& d, int s, int r)
+{
+    if constexpr (!(FM & RIFT_SURGE_GRID)) {
+        return false;
+    } else {
+        if (!(d.flags & RIFT_SURGE_GRID)) return false;
+        if (!d.sg_series) return true;
+        r = min(max(r, 0), d.nurows - 1);
+        return d.sgu[s * d.nurows + r] != 0;
+    }
+}
+
+template <bool Up, unsigned FM>
 __device__ __forceinline__ Lane make_lane(const Dev& d, int s, int rr, int lane)
 {
     Lane L;
@@ -377,24 +441,31 @@ __device__ __forceinline__ Lane make_lane(const Dev& d, int s, int rr, int lane)
     const int TN = (Up ? max(rr - 2, 0) : min(rr + 2, d.nby - 1)) * d.nbx + L.txo;
     L.selN = d.sel[TN] != 0;
     L.actN = d.act[TN] != 0;
+    L.sg0 = unit_surge<FM>(d, s, rr - 1);
+    L.sg1 = unit_surge<FM>(d, s, rr);
+    L.sg2 = unit_surge<FM>(d, s, rr + 1);
+    L.sgN = unit_surge<FM>(d, s, Up ? rr - 2 : rr + 2);
     return L;
 }
 
 // Move the lane's tile-row window to tile row rr = L.r + 1 (down) or - 1.
-template <bool Up>
-__device__ __forceinline__ void shift_lane(const Dev& d, Lane& L, int rr)
+template <bool Up, unsigned FM>
+__device__ __forceinline__ void shift_lane(const Dev& d, Lane& L, int s, int rr)
 {
     L.r = rr;
     if constexpr (!Up) {
         L.sel0 = L.sel1; L.sel1 = L.sel2; L.sel2 = L.selN;
         L.act0 = L.act1; L.act1 = L.act2; L.act2 = L.actN;
+        L.sg0 = L.sg1; L.sg1 = L.sg2; L.sg2 = L.sgN;
     } else {
         L.sel2 = L.sel1; L.sel1 = L.sel0; L.sel0 = L.selN;
         L.act2 = L.act1; L.act1 = L.act0; L.act0 = L.actN;
+        L.sg2 = L.sg1; L.sg1 = L.sg0; L.sg0 = L.sgN;
     }
     const int TN = (Up ? max(rr - 2, 0) : min(rr + 2, d.nby - 1)) * d.nbx + L.txo;
     L.selN = d.sel[TN] != 0;
     L.actN = d.act[TN] != 0;
+    L.sgN = unit_surge<FM>(d, s, Up ? rr - 2 : rr + 2);
 }
 
 
@@ -600,7 +671,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
     int j = Up ? jb - 1 : ja;
     const int jstop = Up ? ja - 1 : jb;
     int rr = (j - 2) / kTileH;
-    Lane L = make_lane<Up>(d, s, rr, lane);
+    Lane L = make_lane<Up, FM>(d, s, rr, lane);
     const float dt = S.dt, inv_dt = S.inv_dt, kappa = S.kappa;
 
     Row R[3];          // finished rows C, D (and E once formed)
@@ -656,7 +727,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             mxl = 0.f;
             myl = 0.f;
             wetl = false;
-            shift_lane<Up>(d, L, rj);
+            shift_lane<Up, FM>(d, L, s, rj);
             rr = rj;
         }
         // prefetch: the row after next and the next output row's inputs
@@ -903,113 +974,166 @@ __host__ __device__ inline int dt_blocks(int ntiles)
     return b < 1 ? 1 : (b > kDtMaxBlocks ? kDtMaxBlocks : b);
 }
 
+// Fixed-order reduction of (max, max, sum, sum) over a workgroup of 256:
+// shuffles within each wave, then lane 0 of the workgroup combines the four
+// waves in order.  Returns the result in thread 0.
+struct DtRed {
+    float vx, vy;
+    unsigned long long cells;
+    int nact;
+};
+__device__ __forceinline__ DtRed dt_block_reduce(DtRed r, DtRed* lds4)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        r.vx = fmaxf(r.vx, __shfl_xor(r.vx, off));
+        r.vy = fmaxf(r.vy, __shfl_xor(r.vy, off));
+        r.cells += __shfl_xor(r.cells, off);
+        r.nact += __shfl_xor(r.nact, off);
+    }
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        r = lds4[0];
+        for (int w = 1; w < 4; ++w) {
+            r.vx = fmaxf(r.vx, lds4[w].vx);
+            r.vy = fmaxf(r.vy, lds4[w].vy);
+            r.cells += lds4[w].cells;
+            r.nact += lds4[w].nact;
+        }
+    }
+    return r;
+}
+
 __device__ __forceinline__ void timestep2_body(const Dev& d)
 {
-    __shared__ float redx[256], redy[256];
-    __shared__ unsigned long long redc[256];
-    __shared__ int redn[256];
-    __shared__ bool last;
-    float vx = 0.f, vy = 0.f;
-    unsigned long long cells = 0ull;
-    int nact = 0;
+    __shared__ DtRed red4[4];
+    __shared__ DtRed total;
+    __shared__ bool last, ended;
+    __shared__ float s_tf;
+    __shared__ int s_match[3];
+    DtRed r{0.f, 0.f, 0ull, 0};
     const int tid = threadIdx.x;
-    // consecutive tiles per workgroup; a grid capped at kDtMaxBlocks strides
+    static_assert(kDtTilesPerBlock == 4 * 256, "four consecutive tiles per lane");
+    // four consecutive tiles per lane: one 32-bit load of the activity bytes
+    // and one 128-bit load of each maxima array, all in flight together (the
+    // per-tile arrays carry 64 bytes of padding, so the last lane's vector
+    // reads stay inside the allocation)
     for (int base = blockIdx.x * kDtTilesPerBlock; base < d.ntiles; base += gridDim.x * kDtTilesPerBlock) {
+        const int t0 = base + 4 * tid;
+        if (t0 < d.ntiles) {
+            const uchar4 a4 = *reinterpret_cast<const uchar4*>(d.act + t0);
+            const float4 x1 = *reinterpret_cast<const float4*>(d.mx1 + t0);
+            const float4 x0 = *reinterpret_cast<const float4*>(d.mx + t0);
+            const float4 y1 = *reinterpret_cast<const float4*>(d.my1 + t0);
+            const float4 y0 = *reinterpret_cast<const float4*>(d.my + t0);
+            const bool a[4] = {a4.x != 0, a4.y != 0, a4.z != 0, a4.w != 0};
+            const float ax1[4] = {x1.x, x1.y, x1.z, x1.w}, ax0[4] = {x0.x, x0.y, x0.z, x0.w};
+            const float ay1[4] = {y1.x, y1.y, y1.z, y1.w}, ay0[4] = {y0.x, y0.y, y0.z, y0.w};
 #pragma unroll
-        for (int k = 0; k < kDtTilesPerBlock / 256; ++k) {
-            const int t = base + k * 256 + tid;
-            if (t >= d.ntiles) break;
-            const bool a = d.act[t] != 0;
-            const float x1 = d.mx1[t], x0 = d.mx[t], y1 = d.my1[t], y0 = d.my[t];
-            vx = fmaxf(vx, a ? x1 : x0);
-            vy = fmaxf(vy, a ? y1 : y0);
-            if (a) {
-                const int bx = t % d.nbx, by = t / d.nbx;
-                const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
-                const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
-                if (i1 > i0 && j1 > j0) cells += (unsigned long long)((i1 - i0) * (j1 - j0));
-                ++nact;
+            for (int k = 0; k < 4; ++k) {
+                const int t = t0 + k;
+                if (t >= d.ntiles) break;
+                r.vx = fmaxf(r.vx, a[k] ? ax1[k] : ax0[k]);
+                r.vy = fmaxf(r.vy, a[k] ? ay1[k] : ay0[k]);
+                if (a[k]) {
+                    const int bx = t % d.nbx, by = t / d.nbx;
+                    const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
+                    const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
+                    if (i1 > i0 && j1 > j0) r.cells += (unsigned long long)((i1 - i0) * (j1 - j0));
+                    ++r.nact;
+                }
             }
         }
     }
-    // fixed-order tree over the workgroup
-    redx[tid] = vx;
-    redy[tid] = vy;
-    redc[tid] = cells;
-    redn[tid] = nact;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) {
-            redx[tid] = fmaxf(redx[tid], redx[tid + w]);
-            redy[tid] = fmaxf(redy[tid], redy[tid + w]);
-            redc[tid] += redc[tid + w];
-            redn[tid] += redn[tid + w];
-        }
-        __syncthreads();
-    }
+    r = dt_block_reduce(r, red4);
     Scalars* sc = d.sc;
     if (tid == 0) {
         DtPartial* p = d.dtp + blockIdx.x;
-        __hip_atomic_store(&p->vx, redx[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->vy, redy[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->cells, redc[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->nact, redn[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->vx, r.vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->vy, r.vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->cells, r.cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->nact, r.nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         last = atomicAdd(&sc->ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last) return;
     __threadfence();
-    vx = vy = 0.f;
-    cells = 0ull;
-    nact = 0;
+    r = DtRed{0.f, 0.f, 0ull, 0};
     for (int b = tid; b < (int)gridDim.x; b += 256) {
         const DtPartial* p = d.dtp + b;
-        vx = fmaxf(vx, __hip_atomic_load(&p->vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        vy = fmaxf(vy, __hip_atomic_load(&p->vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        cells += __hip_atomic_load(&p->cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        nact += __hip_atomic_load(&p->nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        r.vx = fmaxf(r.vx, __hip_atomic_load(&p->vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        r.vy = fmaxf(r.vy, __hip_atomic_load(&p->vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        r.cells += __hip_atomic_load(&p->cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        r.nact += __hip_atomic_load(&p->nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    redx[tid] = vx;
-    redy[tid] = vy;
-    redc[tid] = cells;
-    redn[tid] = nact;
+    __syncthreads();   // red4 is reused
+    r = dt_block_reduce(r, red4);
+    if (tid == 0) total = r;
+    // the tail runs on a copy of the device scalars in LDS, staged and
+    // written back by the whole workgroup (one round trip each way instead
+    // of a chain of dependent single-lane accesses)
+    __shared__ Scalars ssc;
+    constexpr int kWords = (int)(sizeof(Scalars) / sizeof(unsigned));
+    static_assert(sizeof(Scalars) % sizeof(unsigned) == 0, "Scalars is copied in 32-bit words");
+    for (int k = tid; k < kWords; k += 256)
+        reinterpret_cast<unsigned*>(&ssc)[k] = reinterpret_cast<const unsigned*>(sc)[k];
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) {
-            redx[tid] = fmaxf(redx[tid], redx[tid + w]);
-            redy[tid] = fmaxf(redy[tid], redy[tid + w]);
-            redc[tid] += redc[tid + w];
-            redn[tid] += redn[tid + w];
+    if (tid == 0) {
+        ssc.ticket = 0u;
+        ended = ssc.finished != 0;
+        if (ended) {
+            // an ended domain keeps its time; the step's counters are cleared
+            ssc.n_units = 0;
+            ssc.n_units_acc = 0;
+            ssc.next_valid = 0;
+            ssc.b_flux1 = 0.0;
+        } else {
+            ssc.n_active = total.nact;
+            ssc.wet_cells = total.cells;
+            ssc.wcu_total += total.cells;
+            finish_timestep(d, total.vx, total.vy, &ssc);
+            s_tf = (float)ssc.t;
         }
-        __syncthreads();
+        s_match[0] = s_match[1] = s_match[2] = -1;
     }
-    if (tid != 0) return;
-    sc->ticket = 0u;
-    if (sc->finished) {
-        // an ended domain keeps its time; the step's counters are cleared
-        sc->n_units = 0;
-        sc->n_units_acc = 0;
-        sc->next_valid = 0;
-        sc->b_flux1 = 0.0;
+    __syncthreads();
+    if (ended) {
+        for (int k = tid; k < kWords; k += 256)
+            reinterpret_cast<unsigned*>(sc)[k] = reinterpret_cast<const unsigned*>(&ssc)[k];
         return;
     }
-    sc->n_active = redn[0];
-    sc->wet_cells = redc[0];
-    sc->wcu_total += redc[0];
-    finish_timestep(d, redx[0], redy[0]);
-    if (d.flags & RIFT_MASS_BUDGET) {
-        sc->b_out += 0.5 * sc->dt * sc->b_flux1;
-        sc->b_flux1 = 0.0;
+    // the next step's forcing: the workgroup finds the interval of each
+    // series (the last match decides, as in InterpolateRate's loop)
+    const unsigned fl = d.flags;
+    const bool use[3] = {(fl & RIFT_RAIN_AVG) != 0, (fl & RIFT_DRAIN_AVG) != 0, (fl & RIFT_HYDROGRAPH) != 0};
+    const float tf = s_tf;
+    for (int k = 0; k < 3; ++k) {
+        if (!use[k]) continue;
+        const SeriesTable& st = d.ser[k];
+        const int n = st.n;
+        for (int i = tid; i < n - 1; i += 256)
+            if (tf >= st.time[i] && tf < st.time[i + 1]) atomicMax(&s_match[k], i);
     }
-    sc->n_units = sc->n_units_acc;
-    sc->n_units_acc = 0;
-    const Forcing f = forcing_at(d, sc);
-    sc->nxt_rain = f.rain;
-    sc->nxt_drain = f.drain;
-    sc->nxt_hydro = f.hydro;
-    sc->nxt_cell = f.cell;
-    sc->next_valid = 1;
+    __syncthreads();
+    if (tid == 0) {
+        if (fl & RIFT_MASS_BUDGET) {
+            ssc.b_out += 0.5 * ssc.dt * ssc.b_flux1;
+            ssc.b_flux1 = 0.0;
+        }
+        ssc.n_units = ssc.n_units_acc;
+        ssc.n_units_acc = 0;
+        const int match[3] = {s_match[0], s_match[1], s_match[2]};
+        const Forcing f = forcing_at(d, &ssc, match);
+        ssc.nxt_rain = f.rain;
+        ssc.nxt_drain = f.drain;
+        ssc.nxt_hydro = f.hydro;
+        ssc.nxt_cell = f.cell;
+        ssc.next_valid = 1;
+    }
+    __syncthreads();
+    for (int k = tid; k < kWords; k += 256)
+        reinterpret_cast<unsigned*>(sc)[k] = reinterpret_cast<const unsigned*>(&ssc)[k];
 }
 
 __global__ __launch_bounds__(256) void k_timestep2(Dev d)
@@ -1024,8 +1148,8 @@ void launch_timestep2(const Dev& d, hipStream_t s)
 
 void launch_grow2(const Dev& d, hipStream_t s)
 {
-    const int n = d.ntiles + d.nunits;
-    hipLaunchKernelGGL(k_grow2, dim3((n + 255) / 256), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_grow_tiles, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_compact_units, dim3((d.nunits + 255) / 256), dim3(256), 0, s, d);
 }
 
 // Launch variants (all give identical results; they differ in register
@@ -1112,9 +1236,14 @@ void launch_canonicalize(const Dev& d, hipStream_t s)
 // Every member keeps its own time, dt, forcing and tile sets; a member's
 // cells see exactly the arithmetic of a single-domain step.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_grow2_ens(const Dev* __restrict__ devs)
+__global__ __launch_bounds__(256) void k_grow_tiles_ens(const Dev* __restrict__ devs)
 {
-    grow2_body(devs[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
+    grow_tiles_body(devs[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_compact_units_ens(const Dev* __restrict__ devs)
+{
+    compact_units_body(devs[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_timestep2_ens(const Dev* __restrict__ devs)
@@ -1306,9 +1435,10 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
     }
 }
 
-void launch_grow2_ens(const Dev* devs, int nmem, int max_threads, hipStream_t s)
+void launch_grow2_ens(const Dev* devs, int nmem, int max_tiles, int max_units, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_grow2_ens, dim3((max_threads + 255) / 256, nmem), dim3(256), 0, s, devs);
+    hipLaunchKernelGGL(k_grow_tiles_ens, dim3((max_tiles + 255) / 256, nmem), dim3(256), 0, s, devs);
+    hipLaunchKernelGGL(k_compact_units_ens, dim3((max_units + 255) / 256, nmem), dim3(256), 0, s, devs);
 }
 
 void launch_timestep2_ens(const Dev* devs, int nmem, int max_tiles, hipStream_t s)

@@ -380,12 +380,13 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.sel = alloc_bytes<unsigned char>(g, d.ntiles);
     d.flip = alloc_bytes<unsigned char>(g, d.ntiles);
     d.ulist = alloc_bytes<int>(g, d.nunits);
+    d.umark = alloc_bytes<unsigned char>(g, d.nunits);
     d.sc = alloc_bytes<Scalars>(g, 1);
     d.dtp = alloc_bytes<DtPartial>(g, kDtMaxBlocks);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
     ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.dtp && d.ser && g->dsum && d.sel &&
-         d.flip && d.ulist;
+         d.flip && d.ulist && d.umark;
     if (!ok) {
         rift_destroy(g);
         return fail(RIFT_E_HIP, "rift_create: device allocation failed");
@@ -818,6 +819,14 @@ int rift_set_option(rift_grid* g, int option, int value)
     case RIFT_OPT_STEP_PATH:
         if (value != 0 && value != 1) return fail(RIFT_E_ARG, "rift_set_option: step path is 0 or 1");
         g->per_phase = value == 1;
+        if (g->d.sg_series) {
+            // the per-phase kernels do not keep the fused step's per-unit
+            // surge flags: the next fused step recomputes them
+            const int one = 1;
+            HIPCHK(hipMemcpyAsync(&g->d.sc->sg_udirty, &one, sizeof(int), hipMemcpyHostToDevice, g->stream));
+            HIPCHK(hipMemcpyAsync(&g->d.sc->sg_udirty_load, &one, sizeof(int), hipMemcpyHostToDevice, g->stream));
+            HIPCHK(hipStreamSynchronize(g->stream));
+        }
         return RIFT_OK;
     case RIFT_OPT_GRAPHS:
         g->use_graphs = value != 0;
@@ -948,7 +957,8 @@ int rift_map_series_create(rift_grid* g, int which, int nslots, int deltat, doub
     d.sgs = static_cast<float*>(p);
     d.sgflag = alloc_bytes<unsigned char>(g, (size_t)nslots * d.ntiles);
     d.sgtag = alloc_bytes<int>(g, nslots);
-    if (!d.sgflag || !d.sgtag) return fail(RIFT_E_HIP, "rift_map_series_create: device allocation failed");
+    d.sgu = alloc_bytes<unsigned char>(g, d.nunits);
+    if (!d.sgflag || !d.sgtag || !d.sgu) return fail(RIFT_E_HIP, "rift_map_series_create: device allocation failed");
     std::vector<int> tags(nslots, -1);
     HIPCHK(hipMemcpyAsync(d.sgtag, tags.data(), nslots * sizeof(int), hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -984,6 +994,11 @@ int rift_map_series_load(rift_grid* g, int which, int index, const double* host)
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(d.sgtag + slot, &index, sizeof(int), hipMemcpyHostToDevice, g->stream));
     launch_sg_flags(d, slot, g->stream);
+    // the next step recomputes the per-unit surge flags (dt kernels carry
+    // the request over one step, so a load between steps is seen)
+    const int one = 1;
+    HIPCHK(hipMemcpyAsync(&d.sc->sg_udirty, &one, sizeof(int), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(&d.sc->sg_udirty_load, &one, sizeof(int), hipMemcpyHostToDevice, g->stream));
     if ((rc = check_launch("rift_map_series_load"))) return rc;
     HIPCHK(hipStreamSynchronize(g->stream));
     return RIFT_OK;
@@ -1008,6 +1023,7 @@ int rift_map_series_start(rift_grid* g, int which, double t)
     sc.sg_nf = sc.sg_f = (float)factor;
     sc.sg_next_valid = 1;
     sc.sg_missing = -1;
+    sc.sg_udirty = 1;
     HIPCHK(hipMemcpy(g->d.sc, &sc, sizeof(Scalars), hipMemcpyHostToDevice));
     return RIFT_OK;
 }
@@ -1442,7 +1458,7 @@ struct rift_ensemble {
     std::vector<Dev> host_devs;
     unsigned flags = 0;             // TEXTBOOK / MASS_BUDGET (common to all members)
     int march_blocks[2][2] = {{0, 0}, {0, 0}};   // [full|lean][stage]
-    int max_threads = 0, max_tiles = 0;
+    int max_units = 0, max_tiles = 0;
     int min_rows = kTileH;
     std::vector<Dev> uploaded;      // what devs holds (the graphs read it)
     hipGraphExec_t graph = nullptr; // kEnsGraphSteps captured steps
@@ -1460,7 +1476,7 @@ static void ens_step_launches(rift_ensemble* e)
     bool lean = true;
     for (rift_grid* g : e->members) lean = lean && march_lean(g->d.flags);
     const int* blocks = e->march_blocks[lean ? 1 : 0];
-    launch_grow2_ens(e->devs, n, e->max_threads, e->stream);
+    launch_grow2_ens(e->devs, n, e->max_tiles, e->max_units, e->stream);
     launch_march_ens(e->devs, n, e->flags, lean, 1, e->min_rows, blocks[0], e->stream);
     launch_timestep2_ens(e->devs, n, e->max_tiles, e->stream);
     launch_march_ens(e->devs, n, e->flags, lean, 2, e->min_rows, blocks[1], e->stream);
@@ -1505,7 +1521,7 @@ int rift_ensemble_create(int n, const rift_desc* descs, rift_ensemble** out)
         return rc;
     }
     for (rift_grid* g : e->members) {
-        e->max_threads = std::max(e->max_threads, g->d.ntiles + g->d.nunits);
+        e->max_units = std::max(e->max_units, g->d.nunits);
         e->max_tiles = std::max(e->max_tiles, g->d.ntiles);
     }
     const int cus = e->members[0]->cus;
