@@ -1129,12 +1129,16 @@ int rift_download(rift_grid* g, int field, double* host)
         return RIFT_OK;
     }
     if (field == RIFT_W) {
-        if ((rc = get_plane(g, d.h, a)) || (rc = get_plane(g, d.BC, b))) return rc;
+        // over the reference's FP64 cell bed (InitGrid_k's BC), so a dry
+        // cell's surface is exactly its bed and w - BC is never negative for
+        // a reader of this w (a hotstart read by the reference computes
+        // powf(w - BC, 5/3) for Manning, NaN below zero)
+        if ((rc = get_plane(g, d.h, a))) return rc;
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 int bi, bj;
                 surface_bed_cell(nx, ny, i, j, bi, bj);
-                host[(size_t)j * nx + i] = (double)at(a, i, j) + (double)at(b, bi, bj);
+                host[(size_t)j * nx + i] = (double)at(a, i, j) + g->bc_ref[(size_t)bj * nx + bi];
             }
         return RIFT_OK;
     }
@@ -1214,17 +1218,15 @@ int rift_upload(rift_grid* g, int field, const double* host)
     Dev& d = g->d;
     const int nx = d.nx, ny = d.ny;
     if (field == RIFT_W) {
-        // depth = w - BC with the device's cell beds (a ghost's over the
-        // bed of the interior cell its surface is defined on)
-        std::vector<float> bc;
-        int rc = get_plane(g, d.BC, bc);
-        if (rc) return rc;
+        // depth = w - BC over the reference's FP64 cell beds, as downloads
+        // form w (a ghost's over the bed of the interior cell its surface is
+        // defined on)
         std::vector<double> hh((size_t)nx * ny);
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 int bi, bj;
                 surface_bed_cell(nx, ny, i, j, bi, bj);
-                hh[(size_t)j * nx + i] = host[(size_t)j * nx + i] - (double)bc[(size_t)bj * d.P + bi];
+                hh[(size_t)j * nx + i] = host[(size_t)j * nx + i] - g->bc_ref[(size_t)bj * nx + bi];
             }
         return put_field(g, d.h, hh.data(), nx, ny, nx);
     }
