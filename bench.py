@@ -8,8 +8,11 @@ plus fractal noise (amplitude 4 m), Manning map 0.02 below 0 m / 0.06 above,
 rectangle, tile acceleration on, tf = 6 h.  The state is spun up to a fixed
 simulated time (t = 3 h, 5-25 % wet) outside the timed region and
 independent of --warmup; the FP64 volume budget runs over the spin-up and
-its residual is reported.  Then W untimed and K timed steps, inputs resident
-in HBM, stream synchronised on both sides.
+its residual is reported.  The autotuner then picks the launch variant (on
+a copy of the state) and SETTLE_STEPS untimed steps follow it (the step runs
+up to 9 % slower for ~80 steps after the autotuner's handle is freed).  Then
+W untimed and K timed steps, inputs resident in HBM, stream synchronised on
+both sides.
 
 A step is one full Heun time step (both stages, forcing, surge window, tile
 bookkeeping, dt).  A wet-cell update (WCU) is one interior cell of an active
@@ -30,11 +33,13 @@ Second legs: rain basin 2048 x 2048 at full activity, the batched ensemble
 transcription of the same per-cell update (oracle/liboracle32.so) on a
 bounded window of the headline workload.
 
-When run under `rocprofv3 --selected-regions`, only the headline's timed and
-event windows (and the --calibrate copy) are profiled.
+Profiles: the committed summaries run this command under rocprofv3 with
+--graphs 0 (every kernel launched on its own: the same kernels, bit-identical
+results).  `python3 bench.py --profile-summary DIR N` summarises rocprofv3's
+CSV output over the last N dispatches of each kernel; with N = K + min(K,
+100) those are the timed and event windows.
 """
 import argparse
-import ctypes
 import json
 import os
 import platform
@@ -58,6 +63,7 @@ STAGE_BYTES = {
     "rain_basin": {"march1": 40, "march2": 56, "step": 96},
 }
 SPINUP_S = 3 * 3600.0
+SETTLE_STEPS = 128   # untimed steps after the autotuner, before the W warm-up steps
 COASTAL_PROFILE = os.path.join(ROOT, "profiles", "fused_step_coastal_8192.md")
 RAIN_PROFILE = os.path.join(ROOT, "profiles", "fused_step_rain_basin_2048.md")
 
@@ -73,39 +79,6 @@ def _json_default(o):
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
-
-
-class Regions:
-    """roctxProfilerPause/Resume: profiled regions under rocprofv3
-    --selected-regions (no-ops when the library is absent)."""
-
-    def __init__(self):
-        self.lib = None
-        for name in ("librocprofiler-sdk-roctx.so.1", "librocprofiler-sdk-roctx.so"):
-            try:
-                self.lib = ctypes.CDLL(name)
-                break
-            except OSError:
-                continue
-        if self.lib is not None:
-            self.lib.roctxProfilerPause.argtypes = [ctypes.c_uint64]
-            self.lib.roctxProfilerResume.argtypes = [ctypes.c_uint64]
-
-    def pause(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerPause(0)
-
-    def resume(self):
-        if self.lib is not None:
-            self.lib.roctxProfilerResume(0)
-
-
-class NoRegions:
-    def pause(self):
-        pass
-
-    def resume(self):
-        pass
 
 
 def cpu_model():
@@ -172,16 +145,24 @@ def build_rain_basin(n_cells):
     return g, cfg
 
 
-def spin_up(g, t_target, chunk=4096, label="spin-up"):
+def spin_up(g, t_target, chunk=4096, label="spin-up", piece=0):
     """Steps until the simulated time reaches t_target (whole chunks of
-    graph-replayed steps, then single steps); returns the step count."""
+    graph-replayed steps, then single steps); returns the step count.  With
+    piece > 0 each chunk runs as pieces of at most `piece` steps with a
+    stream synchronisation between them (the same steps and states; it
+    bounds the dispatches in flight, for rocprofv3 counter collection)."""
     steps = 0
     t = g.time[0]
     w0 = time.perf_counter()
     while t < t_target:
         _, dt = g.time
         n = max(1, min(chunk, int(0.5 * (t_target - t) / max(dt, 1e-9))))
-        g.step(n)
+        if piece > 0:
+            for k in range(0, n, piece):
+                g.step(min(piece, n - k))
+                g.sync()
+        else:
+            g.step(n)
         steps += n
         t = g.time[0]
         if n == chunk:
@@ -191,7 +172,7 @@ def spin_up(g, t_target, chunk=4096, label="spin-up"):
     return steps
 
 
-def timed_windows(g, steps, warmup, variant, regions, stage_bytes):
+def timed_windows(g, steps, warmup, variant, stage_bytes):
     """W untimed steps, then K timed steps (wall clock between stream
     synchronisations), then a window of min(K, 100) steps with every kernel
     bracketed by HIP events on the handle's stream."""
@@ -201,11 +182,13 @@ def timed_windows(g, steps, warmup, variant, regions, stage_bytes):
     else:
         variant, variant_ms = g.autotune(64)
         log("autotuned launch variant %d: %s ms/step" % (variant, " ".join(f"{m:.4f}" for m in variant_ms)))
+        # after the autotuner (a second handle of the same size created,
+        # stepped and freed) the step runs up to 9 % slower for ~80 steps
+        g.step(SETTLE_STEPS)
     g.step(warmup)
     g.sync()
     t_start, _ = g.time
     _, wcu0 = g.wet_cells()
-    regions.resume()
     t0 = time.perf_counter()
     g.step(steps)
     g.sync()
@@ -216,7 +199,6 @@ def timed_windows(g, steps, warmup, variant, regions, stage_bytes):
     _, wk0 = g.wet_cells()
     g.step(kp_steps)
     g.sync()
-    regions.pause()
     _, wk1 = g.wet_cells()
     stats = {c: g.kernel_stats(c) for c in ("flux1", "stage2_flux", "grow", "timestep")}
     g.profile(False)
@@ -434,13 +416,13 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
             "copies": reps, "ms_per_copy": ms.value / reps}
 
 
-def coastal_leg(args, regions):
+def coastal_leg(args):
     from rift_amd import kp
-    g, cfg = build_coastal(args.cells, budget=True)
+    g, cfg = build_coastal(args.cells, budget=bool(args.spinup_budget))
     if not args.graphs:
         g.set_graphs(False)
-    spin_steps = spin_up(g, SPINUP_S, label="coastal spin-up")
-    b = g.budget()
+    spin_steps = spin_up(g, SPINUP_S, label="coastal spin-up", piece=0 if args.graphs else 32)
+    b = g.budget() if args.spinup_budget else [float("nan")] * 9
     g.set_budget(False)
     t_spin, _ = g.time
     h = g.interior(g.field("depth"))
@@ -451,7 +433,7 @@ def coastal_leg(args, regions):
     log("budget m^3: stored %.6e infiltrated %.3e at reset %.6e rain %.3e source %.6e drain %.3e "
         "outflow %.6e exchange %.6e residual %.3e" % tuple(b))
     sb = STAGE_BYTES["coastal"]
-    w = timed_windows(g, args.steps, args.warmup, args.variant, regions, sb)
+    w = timed_windows(g, args.steps, args.warmup, args.variant, sb)
     state = f"coastal {args.cells} t0={SPINUP_S:.0f} spin={spin_steps} active={n_act0}"
     traffic = measured_traffic(COASTAL_PROFILE, "k_march<2>" if w["dom"] == "march2" else "k_march<1>",
                                state)
@@ -472,10 +454,11 @@ def coastal_leg(args, regions):
         "roofline": roofline(w, sb, traffic),
         "step_roofline": {"bytes_per_wcu": sb["step"], "achieved_GBps": w["value"] * sb["step"] / 1e9,
                           "frac": w["value"] * sb["step"] / HBM_PEAK},
-        "mass_budget": {"window": f"t = 0 .. {t_spin:.0f} s ({spin_steps} steps), FP64 on the device",
-                        "stored_m3": b[0], "source_in_m3": b[4], "outflow_m3": b[6],
-                        "surge_and_clamp_exchange_m3": b[7], "residual_m3": b[8],
-                        "residual_rel": abs(b[8]) / scale},
+        "mass_budget": None if not args.spinup_budget else
+        {"window": f"t = 0 .. {t_spin:.0f} s ({spin_steps} steps), FP64 on the device",
+         "stored_m3": b[0], "source_in_m3": b[4], "outflow_m3": b[6],
+         "surge_and_clamp_exchange_m3": b[7], "residual_m3": b[8],
+         "residual_rel": abs(b[8]) / scale},
     }
     return out
 
@@ -494,7 +477,7 @@ def rain_basin_leg(args):
             break
     log(f"rain basin at full activity after {steps} steps ({n}/{g.ntiles} tiles)")
     sb = STAGE_BYTES["rain_basin"]
-    w = timed_windows(g, max(args.steps, 50), 5, -1, NoRegions(), sb)
+    w = timed_windows(g, max(args.steps, 50), 5, -1, sb)
     g.close()
     return {"value": w["value"], "unit": "WCU/s", "ms_per_step": w["ms_per_step"],
             "workload": "rain_basin 2048x2048 dx=10m, rain + Green-Ampt + Manning, every tile active",
@@ -505,7 +488,83 @@ def rain_basin_leg(args):
             "variant_ms_per_step": w["variant_ms"]}
 
 
+def _profile_rows(root):
+    """Per kernel, the dispatches of rocprofv3 CSV output under `root`, in
+    dispatch order: kernel trace -> [duration ns], counter collection ->
+    {counter: [value]}."""
+    import csv
+    import glob
+    durs, ctrs = {}, {}
+    for path in sorted(glob.glob(os.path.join(root, "**", "*.csv"), recursive=True)):
+        base = os.path.basename(path)
+        if base.endswith("kernel_trace.csv"):
+            with open(path) as fh:
+                rows = sorted(csv.DictReader(fh), key=lambda r: int(r["Dispatch_Id"]))
+            for r in rows:
+                durs.setdefault(r["Kernel_Name"], []).append(
+                    int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        elif base.endswith("counter_collection.csv"):
+            with open(path) as fh:
+                rows = sorted(csv.DictReader(fh), key=lambda r: int(r["Dispatch_Id"]))
+            for r in rows:
+                d = ctrs.setdefault(r["Kernel_Name"], {})
+                d.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    return durs, ctrs
+
+
+def profile_summary(root, last, calib_bytes=256 << 20):
+    """Markdown tables from rocprofv3 output under `root`: per kernel the
+    average duration over all dispatches and over the last `last` (the timed
+    and event windows of a bench run; earlier dispatches are the spin-up and
+    the autotuner), and per counter the average of the last `last`.  With
+    FETCH_SIZE / WRITE_SIZE present, the run's last __amd_rocclr_copyBuffer
+    dispatch (the --calibrate copy of calib_bytes) scales KiB to bytes."""
+    durs, ctrs = _profile_rows(root)
+    out = []
+    if durs:
+        out += ["| kernel | calls | avg us (all) | avg us (last %d) |" % last, "|---|---|---|---|"]
+        for name, d in sorted(durs.items(), key=lambda kv: -sum(kv[1][-last:])):
+            tail = d[-last:]
+            out.append(f"| `{name}` | {len(d)} | {sum(d) / len(d) / 1e3:.2f} | "
+                       f"{sum(tail) / len(tail) / 1e3:.2f} |")
+    scale = {}
+    copy = [k for k in ctrs if "copyBuffer" in k]
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        vals = [v for k in copy for v in ctrs[k].get(c, [])[-1:]]
+        if vals:
+            scale[c] = calib_bytes / (vals[0] * 1024.0)
+            out.append(f"\ncalibration {c}: last copyBuffer dispatch {vals[0]:.0f} KiB for "
+                       f"{calib_bytes} B: factor {scale[c]:.4f}")
+    if ctrs:
+        names = sorted({c for d in ctrs.values() for c in d})
+        out += ["", "| kernel | " + " | ".join(f"{c} (avg of last {last})" for c in names) + " |",
+                "|---|" + "---|" * len(names)]
+        for name, d in sorted(ctrs.items()):
+            if "copyBuffer" in name:
+                continue
+            cells = []
+            for c in names:
+                v = d.get(c, [])[-last:]
+                cells.append(f"{sum(v) / len(v):.0f}" if v else "")
+            out.append(f"| `{name}` | " + " | ".join(cells) + " |")
+        for name, d in sorted(ctrs.items()):
+            if "copyBuffer" in name or not scale:
+                continue
+            nb = 0.0
+            for c, f in scale.items():
+                v = d.get(c, [])[-last:]
+                if v:
+                    nb += sum(v) / len(v) * 1024.0 * f
+            short = name.split("(")[0].replace("void ", "").replace("rift::", "")
+            out.append(f"\nhbm_bytes_per_launch {short}: {nb:.0f}")
+    return "\n".join(out)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--profile-summary":
+        # python3 bench.py --profile-summary DIR LAST: summarise rocprofv3 output (no GPU)
+        print(profile_summary(sys.argv[2], int(sys.argv[3])))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -521,6 +580,9 @@ def main():
     ap.add_argument("--graphs", type=int, default=1,
                     help="1: steps replayed from captured HIP graphs (default); 0: every kernel "
                          "launched on its own (same kernels and results)")
+    ap.add_argument("--spinup-budget", type=int, default=1,
+                    help="1: the FP64 volume budget runs over the spin-up and its residual is "
+                         "reported (default); 0: off (the same states; rocprofv3 --pmc passes use it)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="launch variant of the marching kernels (default: autotuned after the "
                          "spin-up, outside the timed region)")
@@ -535,8 +597,6 @@ def main():
               "(the solver runs one domain per GPU)",
               file=sys.stderr)
         sys.exit(2)
-    regions = Regions()
-    regions.pause()
 
     surr_data = None
     if args.surrogate_train > 0:
@@ -547,7 +607,7 @@ def main():
         surr_data = surrogate.prepare(args.surrogate_train, args.surrogate_test, workers=8)
         log(f"surrogate scenarios generated ({time.perf_counter() - t0:.0f} s)")
 
-    c = coastal_leg(args, regions)
+    c = coastal_leg(args)
     w, g = c["w"], c["g"]
     line = {
         "metric": "wet-cell updates/s",
@@ -597,9 +657,7 @@ def main():
     else:
         line["surrogate"] = None
     if args.calibrate:
-        regions.resume()
         line["calibration_copy_bytes"] = calibration_copy()
-        regions.pause()
     print(json.dumps(line, default=_json_default))
 
 

@@ -26,6 +26,8 @@ __global__ void k_reset_counters(Dev d)
     if (threadIdx.x != 0) return;
     d.sc->n_active = 0;
     d.sc->n_units = 0;
+    d.sc->n_quads = 0;
+    d.sc->n_part = 0;
     d.sc->wet_cells = 0ull;
 }
 
@@ -36,6 +38,8 @@ __global__ void k_begin_step(Dev d)
     sc->n_active = 0;
     sc->n_units = 0;
     sc->n_units_acc = 0;
+    sc->n_quads = sc->n_quads_acc = 0;
+    sc->n_part = sc->n_part_acc = 0;
     sc->wet_cells = 0ull;
     sc->next_valid = 0;
     if (!(d.flags & (RIFT_HYDROGRAPH | RIFT_RAIN_AVG | RIFT_DRAIN_AVG))) return;

@@ -71,6 +71,13 @@ struct Scalars {
     // the per-unit surge flags must be recomputed by the coming step
     // (window moved, or a map was loaded since the last dt kernel)
     int sg_udirty, sg_udirty_load;
+    // single-domain fused step: full quads (all four strips of a quad
+    // active in a tile row) and the active units outside them, compacted by
+    // grow2 (read by stage 1), moved by the dt kernel for stage 2
+    int n_quads, n_quads_acc, n_part, n_part_acc;
+    // fused steps taken (the dt kernel counts); a single domain's grow tags
+    // the units it activates with ustep + 1, so tags never need clearing
+    int ustep;
 };
 
 // One workgroup's share of the fused step's dt reduction.
@@ -100,6 +107,11 @@ struct Dev {
     int* ulist;
     unsigned char* umark;        // per unit: covers an active tile (set by grow, cleared by compaction)
     int nstrips, nurows, nunits;
+    // single domain: qlist = full quad units (q * nurows + r: strips 4q ..
+    // 4q+3 all active in tile row r); ulist then holds the other active units
+    int* qlist;
+    int nquads;
+    int* utag;                   // single domain: step tag of each unit's last activation
     long long soff;              // h0 - h (elements): offset of the second buffer
     int n_uniform;
     float n_const;

@@ -96,6 +96,7 @@ __device__ __forceinline__ bool finished(const Dev& d)
     return d.tf > 0.0 && d.sc->t >= d.tf;
 }
 
+template <bool Mark>
 __device__ __forceinline__ void grow_tiles_body(const Dev& d, int t)
 {
     if (t == 0) {
@@ -135,10 +136,14 @@ __device__ __forceinline__ void grow_tiles_body(const Dev& d, int t)
     const int bx = t % d.nbx, by = t / d.nbx;
     if (by >= d.nurows) return;
     const int s0 = (kTileW * bx) / kStrip, s1 = min((kTileW * bx + kTileW - 1) / kStrip, d.nstrips - 1);
+    const int tag = Mark ? 0 : d.sc->ustep + 1;
     for (int s = s0; s <= s1; ++s) {
         int txa, txb;
         strip_tiles(d, s, txa, txb);
-        if (bx >= txa && bx <= txb) d.umark[s * d.nurows + by] = 1;
+        if (bx >= txa && bx <= txb) {
+            if constexpr (Mark) d.umark[s * d.nurows + by] = 1;
+            else d.utag[s * d.nurows + by] = tag;
+        }
     }
 }
 
@@ -180,14 +185,56 @@ __device__ __forceinline__ void compact_units_body(const Dev& d, int u)
     if (ua) d.ulist[ubase + __popcll(mu & lt)] = u;
 }
 
-__global__ __launch_bounds__(256) void k_grow_tiles(Dev d)
+// Single domain, one thread per unit u = (s, r), strip-major, after every
+// tile is tagged: the unit is active when grow tagged it in this step; its
+// quad (strips 4q .. 4q+3, q = s / 4) is full when all four strips are
+// active in tile row r.  A full quad is listed once (by its first strip's
+// thread) in qlist, the other active units in ulist; runs of one strip stay
+// in order within each wave's 64 units.
+
+__device__ __forceinline__ void compact_quads_body(const Dev& d, int u)
 {
-    grow_tiles_body(d, blockIdx.x * blockDim.x + threadIdx.x);
+    bool own = false, full = false;
+    int s = 0, r = 0;
+    if (u < d.nunits) {
+        s = u / d.nurows;
+        r = u - s * d.nurows;
+        if (!finished(d)) {
+            const int tag = d.sc->ustep + 1;
+            own = d.utag[u] == tag;
+            const int s0 = s & ~3;
+            if (own && s0 + 3 < d.nstrips) {
+                full = true;
+                for (int k = s0; k < s0 + 4; ++k) full = full && d.utag[k * d.nurows + r] == tag;
+            }
+        }
+        if ((d.flags & RIFT_SURGE_GRID) && d.sg_series && d.sc->sg_udirty) unit_surge_flags(d, u, s, r);
+    }
+    const bool part = own && !full;
+    const bool head = full && (s & 3) == 0;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const unsigned long long mp = __ballot(part), mh = __ballot(head), mo = __ballot(own);
+    int pbase = 0, hbase = 0;
+    if (lane == 0) {
+        if (mo) atomicAdd(&d.sc->n_units_acc, __popcll(mo));
+        if (mp) pbase = atomicAdd(&d.sc->n_part_acc, __popcll(mp));
+        if (mh) hbase = atomicAdd(&d.sc->n_quads_acc, __popcll(mh));
+    }
+    pbase = __shfl(pbase, 0);
+    hbase = __shfl(hbase, 0);
+    if (part) d.ulist[pbase + __popcll(mp & lt)] = u;
+    if (head) d.qlist[hbase + __popcll(mh & lt)] = (s >> 2) * d.nurows + r;
 }
 
-__global__ __launch_bounds__(256) void k_compact_units(Dev d)
+__global__ __launch_bounds__(256) void k_grow_tiles(Dev d)
 {
-    compact_units_body(d, blockIdx.x * blockDim.x + threadIdx.x);
+    grow_tiles_body<false>(d, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_compact_quads(Dev d)
+{
+    compact_quads_body(d, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -834,9 +881,19 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
 // shares meet at a boundary either both at their first rows or both at
 // their last, so the halo rows around it are read once from HBM.
 //
-// Otherwise: the active units (strip-major) form a sequence of rows and wave
-// i marches the i-th equal share of at least Dev::min_rows rows, downwards,
-// continuing from one unit into the next while they are vertically adjacent.
+// Otherwise (sparse): full quads (four neighbouring strips all active in a
+// tile row) go to workgroups, the other active units to single waves, the
+// workgroups split between the two in proportion to their rows.  Each list
+// forms a sequence of rows; workgroup (wave) i marches the i-th equal share
+// of at least Dev::min_rows rows, downwards, continuing from one entry into
+// the next while they are vertically adjacent.  The four waves of a quad's
+// workgroup march the same rows of its four strips side by side on one CU,
+// so the 128-byte lines two neighbouring strips' windows share (a strip's
+// 64-column window starts 240 bytes after the last one's) come from HBM
+// once: dealt one strip per wave, every wave read 2.8 lines a row for the
+// 240 bytes of its 60 output columns.  Partial quads stay with single waves:
+// a workgroup per quad idled 19 % of its waves on the coastal-surge
+// workload at 3 h and ran 9 % slower despite 21 % less traffic.
 template <int Stage, bool Textbook, int Waves, bool Budget, int Kind>
 __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
 {
@@ -888,31 +945,47 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         return;
     }
 
-    const long long total = (long long)nunits * kTileH;
-    const long long nshare = max(1LL, min((long long)W, total / max(d.min_rows, 1)));
-    // shares [i0, i1) of this wave (at most one when nshare <= W)
-    const long long i0 = ((long long)wp * nshare + W - 1) / W;
-    const long long i1 = ((long long)(wp + 1) * nshare + W - 1) / W;
+    // full quads to the first GA workgroups, the other active units to the
+    // waves of the rest, in proportion to their rows
+    const int nqf = (Stage == 1) ? d.sc->n_quads_acc : d.sc->n_quads;
+    const int npart = (Stage == 1) ? d.sc->n_part_acc : d.sc->n_part;
+    const long long ua = 4LL * nqf, ub = npart;
+    int GA = (int)((G * ua + (ua + ub) / 2) / max(ua + ub, 1LL));
+    if (ua > 0) GA = max(GA, 1);
+    if (ub > 0) GA = min(GA, G - 1);
+    const bool quad = pos < GA;
+    const int* list = quad ? d.qlist : d.ulist;
+    const int nentries = quad ? nqf : npart;
+    const int nwork = quad ? GA : 4 * (G - GA);   // workgroups / waves sharing it
+    const int me = quad ? pos : 4 * (pos - GA) + wid;
+    const long long total = (long long)nentries * kTileH;
+    const long long nshare = max(1LL, min((long long)nwork, total / max(d.min_rows, 1)));
+    // shares [i0, i1) of this workgroup / wave (at most one when nshare <= nwork)
+    const long long i0 = ((long long)me * nshare + nwork - 1) / nwork;
+    const long long i1 = ((long long)(me + 1) * nshare + nwork - 1) / nwork;
     if (i0 >= i1) return;
     const long long p0 = total * i0 / nshare, pend = total * i1 / nshare;
     if (p0 >= pend) return;
     int k = (int)(p0 / kTileH);
     int a = (int)(p0 - (long long)k * kTileH);
     for (;;) {
-        // a maximal run of vertically adjacent units of one strip
-        const int u = __builtin_amdgcn_readfirstlane(nunits == d.nunits ? k : d.ulist[k]);
-        const int s = u / d.nurows, rr = u - s * d.nurows;
+        // a maximal run of vertically adjacent entries (units of one strip,
+        // or quads of one column of quads)
+        const int u = __builtin_amdgcn_readfirstlane(list[k]);
+        const int c = u / d.nurows, rr = u - c * d.nurows;
         const int ja = rr * kTileH + 2 + a;
         int last = u, kk = k;
         while ((long long)(kk + 1) * kTileH < pend) {
-            const int un = __builtin_amdgcn_readfirstlane(nunits == d.nunits ? kk + 1 : d.ulist[kk + 1]);
-            if (un != last + 1 || un - s * d.nurows <= 0 || un - s * d.nurows >= d.nurows) break;
+            const int un = __builtin_amdgcn_readfirstlane(list[kk + 1]);
+            if (un != last + 1 || un - c * d.nurows <= 0 || un - c * d.nurows >= d.nurows) break;
             last = un;
             ++kk;
         }
-        const int rlast = last - s * d.nurows;
+        const int rlast = last - c * d.nurows;
         const long long pe = min(pend, (long long)(kk + 1) * kTileH);
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (long long)kk * kTileH), d.ny - 2);
+        // a quad's wave w marches strip 4c + w
+        const int s = quad ? 4 * c + wid : c;
         if (ja < jb) march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
         k = kk + 1;
         a = 0;
@@ -1086,6 +1159,8 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
             // an ended domain keeps its time; the step's counters are cleared
             ssc.n_units = 0;
             ssc.n_units_acc = 0;
+            ssc.n_quads = ssc.n_quads_acc = 0;
+            ssc.n_part = ssc.n_part_acc = 0;
             ssc.next_valid = 0;
             ssc.b_flux1 = 0.0;
         } else {
@@ -1123,6 +1198,11 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
         }
         ssc.n_units = ssc.n_units_acc;
         ssc.n_units_acc = 0;
+        ssc.n_quads = ssc.n_quads_acc;
+        ssc.n_quads_acc = 0;
+        ssc.n_part = ssc.n_part_acc;
+        ssc.n_part_acc = 0;
+        ssc.ustep += 1;
         const int match[3] = {s_match[0], s_match[1], s_match[2]};
         const Forcing f = forcing_at(d, &ssc, match);
         ssc.nxt_rain = f.rain;
@@ -1149,7 +1229,7 @@ void launch_timestep2(const Dev& d, hipStream_t s)
 void launch_grow2(const Dev& d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_grow_tiles, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_compact_units, dim3((d.nunits + 255) / 256), dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_compact_quads, dim3((d.nunits + 255) / 256), dim3(256), 0, s, d);
 }
 
 // Launch variants (all give identical results; they differ in register
@@ -1238,7 +1318,7 @@ void launch_canonicalize(const Dev& d, hipStream_t s)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_grow_tiles_ens(const Dev* __restrict__ devs)
 {
-    grow_tiles_body(devs[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
+    grow_tiles_body<true>(devs[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_compact_units_ens(const Dev* __restrict__ devs)

@@ -305,6 +305,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.nstrips = (d.nx - 4 + kStrip - 1) / kStrip;
     d.nurows = (d.ny - 4 + kTileH - 1) / kTileH;
     d.nunits = d.nstrips * d.nurows;
+    d.nquads = (d.nstrips + 3) / 4;
     // the pitch covers the 20-column window of the last tile and the 64
     // lanes of the last strip
     d.P = ((std::max(kTileW * d.nbx + 4, kStrip * d.nstrips + 4) + 63) / 64) * 64;
@@ -381,12 +382,14 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.flip = alloc_bytes<unsigned char>(g, d.ntiles);
     d.ulist = alloc_bytes<int>(g, d.nunits);
     d.umark = alloc_bytes<unsigned char>(g, d.nunits);
+    d.qlist = alloc_bytes<int>(g, (size_t)d.nquads * d.nurows);
+    d.utag = alloc_bytes<int>(g, d.nunits);
     d.sc = alloc_bytes<Scalars>(g, 1);
     d.dtp = alloc_bytes<DtPartial>(g, kDtMaxBlocks);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
     ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.dtp && d.ser && g->dsum && d.sel &&
-         d.flip && d.ulist && d.umark;
+         d.flip && d.ulist && d.umark && d.qlist && d.utag;
     if (!ok) {
         rift_destroy(g);
         return fail(RIFT_E_HIP, "rift_create: device allocation failed");
@@ -708,6 +711,9 @@ static int capture_steps(rift_grid* g, int n, hipGraphExec_t* out)
     e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(RIFT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    // upload now, so that the first replay does not pay for it
+    e = hipGraphUpload(*out, g->stream);
+    if (e != hipSuccess) return fail(RIFT_E_HIP, std::string("hipGraphUpload: ") + hipGetErrorString(e));
     return RIFT_OK;
 }
 
