@@ -9,8 +9,9 @@ rectangle, tile acceleration on, tf = 6 h.  The state is spun up to a fixed
 simulated time (t = 3 h, 5-25 % wet) outside the timed region and
 independent of --warmup; the FP64 volume budget runs over the spin-up and
 its residual is reported.  The autotuner then picks the launch variant (on
-a copy of the state) and SETTLE_STEPS untimed steps follow it (the step runs
-up to 9 % slower for ~80 steps after the autotuner's handle is freed).  Then
+a copy of the state) and SETTLE_STEPS untimed steps follow (the step runs up
+to 9 % slower for ~80 steps after the autotuner's handle is freed; they run
+with --variant too, so that every run reaches the same state).  Then
 W untimed and K timed steps, inputs resident in HBM, stream synchronised on
 both sides.
 
@@ -91,21 +92,26 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def measured_traffic(path, kernel, state):
+def measured_traffic(path, kernel, state, variant):
     """HBM bytes per launch of `kernel` (rocprofv3 FETCH_SIZE / WRITE_SIZE of
     this command, calibrated against a copy of known size) from a committed
     profile summary, when the summary was taken on the same state (same
-    `state` key: workload, simulated time and active tiles); else None."""
+    `state` key: workload, spin-up, active tiles and start of the timed
+    window) with a launch variant of the same traffic class (a line
+    `traffic_bytes_per_launch <kernel> variants a,b,..: bytes` listing it);
+    else None."""
     if not os.path.exists(path):
         return None
     want = f"state: {state}"
     lines = [ln.rstrip("\n") for ln in open(path)]
     if want not in lines:
         return None
-    key = f"traffic_bytes_per_launch {kernel}:"
+    key = f"traffic_bytes_per_launch {kernel} variants "
     for ln in lines:
         if ln.startswith(key):
-            return float(ln[len(key):].strip())
+            vs, val = ln[len(key):].split(":", 1)
+            if variant in {int(v) for v in vs.split(",")}:
+                return float(val.strip())
     return None
 
 
@@ -182,9 +188,10 @@ def timed_windows(g, steps, warmup, variant, stage_bytes):
     else:
         variant, variant_ms = g.autotune(64)
         log("autotuned launch variant %d: %s ms/step" % (variant, " ".join(f"{m:.4f}" for m in variant_ms)))
-        # after the autotuner (a second handle of the same size created,
-        # stepped and freed) the step runs up to 9 % slower for ~80 steps
-        g.step(SETTLE_STEPS)
+    # after the autotuner (a second handle of the same size created, stepped
+    # and freed) the step runs up to 9 % slower for ~80 steps; the settle
+    # steps run with a given variant too, so that both reach the same state
+    g.step(SETTLE_STEPS)
     g.step(warmup)
     g.sync()
     t_start, _ = g.time
@@ -434,9 +441,10 @@ def coastal_leg(args):
         "outflow %.6e exchange %.6e residual %.3e" % tuple(b))
     sb = STAGE_BYTES["coastal"]
     w = timed_windows(g, args.steps, args.warmup, args.variant, sb)
-    state = f"coastal {args.cells} t0={SPINUP_S:.0f} spin={spin_steps} active={n_act0}"
+    state = (f"coastal {args.cells} t0={SPINUP_S:.0f} spin={spin_steps} active={n_act0} "
+             f"settle={SETTLE_STEPS} warmup={args.warmup} t_timed={w['t_start']:.4f}")
     traffic = measured_traffic(COASTAL_PROFILE, "k_march<2>" if w["dom"] == "march2" else "k_march<1>",
-                               state)
+                               state, w["variant"])
     k0, k1, f, miss = g.map_state(kp.MAP_SURGE)
     scale = max(abs(b[0]) + abs(b[1]), abs(b[2]), abs(b[4]), abs(b[6]), 1e-300)
     out = {

@@ -25,12 +25,17 @@ def test_traffic_only_for_the_profiled_state(tmp_path):
     import bench
     p = tmp_path / "summary.md"
     p.write_text("# summary\nstate: coastal 64 t0=10800 spin=5 active=7\n"
-                 "traffic_bytes_per_launch k_march<2>: 1234.5\n")
-    assert bench.measured_traffic(str(p), "k_march<2>", "coastal 64 t0=10800 spin=5 active=7") == 1234.5
-    # another state (other active tile count) or another kernel: no figure
-    assert bench.measured_traffic(str(p), "k_march<2>", "coastal 64 t0=10800 spin=5 active=8") is None
-    assert bench.measured_traffic(str(p), "k_march<1>", "coastal 64 t0=10800 spin=5 active=7") is None
-    assert bench.measured_traffic(str(tmp_path / "absent.md"), "k_march<2>", "x") is None
+                 "traffic_bytes_per_launch k_march<2> variants 4,5,12,13: 1234.5\n"
+                 "traffic_bytes_per_launch k_march<2> variants 0,1,8,9: 1300\n")
+    st = "coastal 64 t0=10800 spin=5 active=7"
+    assert bench.measured_traffic(str(p), "k_march<2>", st, 5) == 1234.5
+    assert bench.measured_traffic(str(p), "k_march<2>", st, 9) == 1300.0
+    # a variant of an unprofiled class, another state (other active tile
+    # count) or another kernel: no figure
+    assert bench.measured_traffic(str(p), "k_march<2>", st, 2) is None
+    assert bench.measured_traffic(str(p), "k_march<2>", "coastal 64 t0=10800 spin=5 active=8", 5) is None
+    assert bench.measured_traffic(str(p), "k_march<1>", st, 5) is None
+    assert bench.measured_traffic(str(tmp_path / "absent.md"), "k_march<2>", "x", 5) is None
 
 
 def test_stage_bytes_add_up_to_the_step():
