@@ -288,7 +288,7 @@ def coastal_cpu_baseline(cfg, g, budget_s=20.0, max_steps=60):
                       f"(oracle/liboracle32.so), {el:.1f} s, {threads} threads ({setup:.0f} s set-up)"}
 
 
-def ensemble_leg(nscen, seed=4, check_every=64):
+def ensemble_leg(nscen, seed=4, check_every=64, graphs=True):
     """The batched-ensemble workload (rift_amd.synth.ensemble_scenarios: sides
     512..2048, dx = 10 m, tf = 1 h, per-scenario rain peak, Manning n and
     Green-Ampt K) advanced to every scenario's end time by rift_ensemble_run;
@@ -305,6 +305,8 @@ def ensemble_leg(nscen, seed=4, check_every=64):
         t, v = synth.series_hours_to_seconds(m["hours"], m["mmhr"])
         g.set_series(kp.SER_HYETO, t, v)
         e.set_tf(i, m["tf"])
+        if not graphs:
+            g.set_graphs(False)
     cells = sum(m["side"] ** 2 for m in scen)
     del scen, spec
     setup_s = time.perf_counter() - t_gen
@@ -330,13 +332,11 @@ def ensemble_leg(nscen, seed=4, check_every=64):
 
 
 def surrogate_leg(data, epochs):
-    """Surrogate screening (rift_amd.surrogate.run) at reduced size: the
-    network of the full workload (4 levels, 32 base channels, BF16 on the
-    matrix cores) trained on len(train) solver runs at 256 x 256 (dx = 10 m,
-    tf = 1 h), then screening len(test) new scenarios with solver fallback
-    within 0.1 m of a 0.3 m asset-depth threshold and 5 % held out.  The full
-    workload (4000 training runs, 2048 test scenarios) is
-    `python -m rift_amd.surrogate`."""
+    """Surrogate screening (rift_amd.surrogate.run): the U-Net (4 levels, 32
+    base channels, BF16 on the matrix cores) trained on len(train) solver
+    runs at 256 x 256 (dx = 10 m, tf = 1 h; 4000 by default, the full
+    workload), then screening len(test) new scenarios (2048) with solver
+    fallback within 0.1 m of a 0.3 m asset-depth threshold and 5 % held out."""
     from rift_amd import surrogate
     r = surrogate.run(epochs=epochs, data=data,
                       log=lambda m: print("[bench] surrogate: " + m, file=sys.stderr, flush=True))
@@ -352,7 +352,7 @@ def surrogate_leg(data, epochs):
                        "achieved": st["train_tflops"], "peak": surrogate.BF16_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": st["train_tflops"] / surrogate.BF16_PEAK_TFLOPS}
     out["workload"] = (f"U-Net 4 levels x 32 base ch, BF16 MFMA; {r['n_train']} training runs / "
-                       f"{r['n_test']} test scenarios at 256x256 (reduced from 4000 / 2048), "
+                       f"{r['n_test']} test scenarios at 256x256, "
                        "threshold 0.3 m +- 0.1 m at the central 8x8 cells, 5% held out")
     return out
 
@@ -594,11 +594,11 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="launch variant of the marching kernels (default: autotuned after the "
                          "spin-up, outside the timed region)")
-    ap.add_argument("--surrogate-train", type=int, default=512,
+    ap.add_argument("--surrogate-train", type=int, default=4000,
                     help="training scenarios of the surrogate-screening leg (0: skip the leg)")
-    ap.add_argument("--surrogate-test", type=int, default=512,
+    ap.add_argument("--surrogate-test", type=int, default=2048,
                     help="test scenarios of the surrogate-screening leg")
-    ap.add_argument("--surrogate-epochs", type=int, default=12)
+    ap.add_argument("--surrogate-epochs", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -655,7 +655,7 @@ def main():
         line["rain_basin_2048"] = rain_basin_leg(args)
     if args.ensemble > 0:
         log("ensemble ...")
-        line["ensemble"] = ensemble_leg(args.ensemble)
+        line["ensemble"] = ensemble_leg(args.ensemble, graphs=bool(args.graphs))
     else:
         line["ensemble"] = None
     if surr_data is not None:

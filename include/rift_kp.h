@@ -186,7 +186,8 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
  * Integrate_1, ApplyBoundaries, ...); both give the same results. */
 #define RIFT_OPT_STEP_PATH 1
 /* RIFT_OPT_GRAPHS: 1 (default) = replay the fused step from captured HIP
- * graphs (16 steps per launch), 0 = launch its kernels one by one. */
+ * graphs (16 steps per launch), 0 = launch its kernels one by one (an
+ * ensemble with any member at 0 launches its steps one by one too). */
 #define RIFT_OPT_GRAPHS 2
 /* RIFT_OPT_VARIANT: launch variant of the fused step's marching kernels,
  * 0 .. RIFT_NVARIANTS-1 (default 0): bit 0 = stage 1 with a larger register

@@ -1582,6 +1582,14 @@ int rift_ensemble_step(rift_ensemble* e, int nsteps)
         e->uploaded = e->host_devs;
     }
     int left = nsteps;
+    // graph replays unless a member has graphs switched off (RIFT_OPT_GRAPHS)
+    bool graphs = true;
+    for (rift_grid* g : e->members) graphs = graphs && g->use_graphs;
+    if (!graphs) {
+        for (; left > 0; --left) ens_step_launches(e);
+        for (rift_grid* g : e->members) g->scattered = true;
+        return check_launch("rift_ensemble_step");
+    }
     if (left >= kEnsGraphSteps && !e->graph) {
         // the ensemble's step loop as one graph (no memset nodes)
         hipGraph_t graph = nullptr;

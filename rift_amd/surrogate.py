@@ -198,13 +198,14 @@ def _gen(args):
     return sc, features(sc)
 
 
-def generate(seeds, side=SIDE, tf=TF, workers=1):
+def generate(seeds, side=SIDE, tf=TF, workers=1, start="fork"):
     """(scenarios, features) for the seeds; `workers` > 1 uses a process pool
-    (create it before the process touches the GPU)."""
+    (forked: create it before the process touches the GPU; start="spawn"
+    starts fresh interpreters, for a process that already has)."""
     args = [(int(s), side, tf) for s in seeds]
     if workers > 1:
         import multiprocessing as mp
-        with mp.get_context("fork").Pool(workers) as pool:
+        with mp.get_context(start).Pool(workers) as pool:
             res = pool.map(_gen, args, chunksize=8)
     else:
         res = [_gen(a) for a in args]
@@ -411,10 +412,11 @@ def screen(net, scs, X, heldout_frac=0.05, seed=1, chunk=256, log=None):
             "errors": err}
 
 
-def prepare(n_train, n_test, train_seed0=5000, test_seed0=9000, workers=16, side=SIDE, tf=TF):
+def prepare(n_train, n_test, train_seed0=5000, test_seed0=9000, workers=16, side=SIDE, tf=TF,
+            start="fork"):
     """Training and test scenarios with their features (host only)."""
-    scs_tr, Xtr = generate(range(train_seed0, train_seed0 + n_train), side, tf, workers)
-    scs_te, Xte = generate(range(test_seed0, test_seed0 + n_test), side, tf, workers)
+    scs_tr, Xtr = generate(range(train_seed0, train_seed0 + n_train), side, tf, workers, start)
+    scs_te, Xte = generate(range(test_seed0, test_seed0 + n_test), side, tf, workers, start)
     return scs_tr, Xtr, scs_te, Xte
 
 

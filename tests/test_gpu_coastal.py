@@ -21,7 +21,7 @@ hotstart (surge series restarted at that time with the maps around it; the
 FP32 values given low bits, _generic) and
 both advance side by side: dt within 5e-3 relative, depth within 1e-4 m
 after the window, and tile flags equal except on thin-front tiles
-(_lockstep_thin_front).  Launch variants of the coastal instantiation stay
+(lockstep_lib).  Launch variants of the coastal instantiation stay
 bit-identical.
 """
 import numpy as np
@@ -29,6 +29,7 @@ import pytest
 
 import bench
 import oracle_lib as O
+from lockstep_lib import check_tiles
 from rift_amd import synth
 
 pytestmark = pytest.mark.gpu
@@ -69,47 +70,10 @@ def test_coastal_8192_budget_positivity_and_tile_rule(rift):
     g.close()
 
 
-THIN = 1e-5   # m: the wettest cell of a tile whose flag may differ
-
-
-def _tile_max_depth(h, nbx, nby):
-    # per-tile maximum depth over each 16 x 14 tile's interior cells
-    ny, nx = h.shape
-    out = np.zeros(nbx * nby)
-    for by in range(nby):
-        j0, j1 = by * 14 + 2, min(by * 14 + 16, ny - 2)
-        if j1 <= j0:
-            continue
-        rows = h[j0:j1]
-        for bx in range(nbx):
-            i0, i1 = bx * 16 + 2, min(bx * 16 + 18, nx - 2)
-            if i1 > i0:
-                out[by * nbx + bx] = rows[:, i0:i1].max()
-    return out
-
-
-def _dilate(tiles, nbx, nby):
-    out = set()
-    for t in tiles:
-        bx, by = t % nbx, t // nbx
-        out.add(t)
-        for ddx, ddy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
-            x, y = bx + ddx, by + ddy
-            if 0 <= x < nbx and 0 <= y < nby:
-                out.add(y * nbx + x)
-    return out
-
-
 def _lockstep_thin_front(o, g, nsteps):
     """Lockstep of the device and the FP64 oracle on a workload with a
-    dry-land flood front.  dt agrees to 5e-3 relative.  The wet flag of a
-    tile is bit-exact given the same depths (the Track/Grow rule is checked
-    on the device's own flags in the 8192^2 test); here the depths are FP32
-    against FP64, and a front cell the oracle leaves at 2e-7 m FP32 may put
-    at or below epsilon (1.19e-7 m).  So flags may differ only on tiles
-    whose wettest cell is below THIN in both, and active sets only where the
-    Grow rule propagates such a difference from the step before."""
-    nbx, nby = g.nbx, g.nby
+    dry-land flood front: dt within 5e-3 relative, tile flags compared by
+    lockstep_lib.check_tiles."""
     dt_err, nthin, prev = 0.0, 0, set()
     for k in range(nsteps):
         o.step()
@@ -118,21 +82,8 @@ def _lockstep_thin_front(o, g, nsteps):
         tg, dtg = g.time
         dt_err = max(dt_err, abs(dtg - dto) / dto)
         assert abs(dtg - dto) <= 5e-3 * dto, (k, dto, dtg)
-        gw, ga, _ = g.tiles()
-        ow, oa, _ = o.tiles()
-        oact = np.zeros(g.ntiles, np.uint8)
-        oact[[int(v) for v in oa if v > -1]] = 1
-        oact[0] = 1
-        wd = set(np.nonzero(gw != ow)[0].tolist())
-        if wd:
-            hg = _tile_max_depth(g.field("depth"), nbx, nby)
-            ho = _tile_max_depth(o.depth(), nbx, nby)
-            for t in wd:
-                assert hg[t] < THIN and ho[t] < THIN, (k, t, hg[t], ho[t])
-        ad = set(np.nonzero(ga != oact)[0].tolist())
-        assert ad <= _dilate(prev, nbx, nby), (k, sorted(ad - _dilate(prev, nbx, nby))[:10])
-        nthin += len(wd)
-        prev = wd
+        prev = check_tiles(g, o, prev, k)
+        nthin += len(prev)
     return nsteps, dt_err, nthin
 
 
