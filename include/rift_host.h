@@ -34,6 +34,17 @@ int rift_geometry(int b_nx, int b_ny, const double* dem, const double* h0c, cons
  * at the DEM corners (b_ny x b_nx, row 0 = south): the four-corner mean
  * where more than two corners differ from nodata, times scale; every other
  * cell holds nodata when allow_nodata, else 0. */
+/* Corner elevations behind the face beds (no reference counterpart: the
+ * reference keeps only the face means, simulator.cu:384-414).  BX/BY as
+ * rift_geometry writes them (nx = b_nx+3, ny = b_ny+3) are two-point means
+ * of the corners D (b_ny x b_nx, row 0 = south); the means fix D up to a
+ * checkerboard (-1)^(i+j) c, which changes no mean, and the D returned has
+ * the checkerboard's mean taken out.  Returns RIFT_E_ARG (D untouched) when
+ * the faces are not such means within 1e-9 relative or their ghost faces do
+ * not repeat the faces two cells in.  The fused step forms every bed from
+ * these corners. */
+int rift_corners_from_faces(int nx, int ny, const double* BX, const double* BY, double* D);
+
 int rift_map_cells(int b_nx, int b_ny, const double* corners, double scale, int allow_nodata,
                    double nodata, double* cells);
 

@@ -196,6 +196,16 @@ __device__ __forceinline__ float green_ampt(float K, float F, float psi_dtheta, 
     return (rate > 0.f) ? rate : 0.f;
 }
 
+// Beds from the corner elevations (the DEM points, simulator.cu:384-414):
+// with p(j) = D(c, j) + D(c+1, j) the pair sum of corner row j, the cell
+// bed is (p(j) + p(j+1))/4 (BC, the mean of the four face beds, grid.cu:
+// 531-537), the south face (p(j))/2 (BY) and the west face
+// (D(c, j) + D(c, j+1))/2 (BX).  Every kernel forms them with these
+// expressions, so that all paths see the same FP32 beds.
+__device__ __forceinline__ float bed_cell(float p_lo, float p_hi) { return 0.25f * (p_lo + p_hi); }
+__device__ __forceinline__ float bed_south(float p_lo) { return 0.5f * p_lo; }
+__device__ __forceinline__ float bed_west(float d_lo, float d_hi) { return 0.5f * (d_lo + d_hi); }
+
 // Critical depth (q^2/g)^(1/3) of the outflow boundary (grid.cu:1342).
 __device__ __forceinline__ float critical_depth(float q)
 {

@@ -694,6 +694,30 @@ void launch_interp_map(const Dev& d, int which, float factor, hipStream_t s)
     hipLaunchKernelGGL(k_interp_map, grid, dim3(256), 0, s, d, which, factor);
 }
 
+// BC, BX and BY of every plane cell from the corner plane (period-2 beyond
+// the corners, so that ghost faces repeat the faces two cells in and ghost
+// cells take the bed of the interior cell they copy from): the per-phase
+// kernels read the same FP32 beds the fused step forms from the corners.
+__global__ __launch_bounds__(256) void k_beds_from_corners(Dev d, int R)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i >= d.P || j >= R) return;
+    const int i1 = (i + 1 < d.P) ? i + 1 : i - 1;
+    const int j1 = (j + 1 < R) ? j + 1 : j - 1;
+    const float* r0 = d.DC + (size_t)j * d.P;
+    const float* r1 = d.DC + (size_t)j1 * d.P;
+    const float p0 = r0[i] + r0[i1], p1 = r1[i] + r1[i1];
+    const size_t q = (size_t)j * d.P + i;
+    d.BC[q] = bed_cell(p0, p1);
+    d.BY[q] = bed_south(p0);
+    d.BX[q] = bed_west(r0[i], r1[i]);
+}
+
+void launch_beds_from_corners(const Dev& d, int R, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_beds_from_corners, dim3((d.P + 255) / 256, R), dim3(256), 0, s, d, R);
+}
+
 void launch_sum_interior(const Dev& d, const float* x, int mode, double* out, hipStream_t s)
 {
     (void)mode;

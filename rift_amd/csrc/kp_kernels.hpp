@@ -97,6 +97,9 @@ struct SurgeWindow {
 struct Dev {
     float *h, *hu, *hv, *h0, *hu0, *hv0;
     float *dh, *dhu, *dhv, *G, *F, *F0, *dF, *K, *n, *BC, *BX, *BY;
+    // corner elevations behind the face beds (the DEM points; the fused
+    // step forms BC, BX and BY from them, see k_beds_from_corners)
+    float* DC;
     float *map[3][3];            // [rain|drain|surge][current|plane1|plane2]
     float *hmax, *qmax, *twet, *tpeak, *tdry;
     unsigned char *wet, *act;
@@ -315,6 +318,8 @@ void launch_sg_flags(const Dev& d, int slot, hipStream_t s);
 void launch_track_series(const Dev& d, hipStream_t s);
 void launch_sg_materialize(const Dev& d, hipStream_t s);
 void launch_sum_interior(const Dev& d, const float* x, int mode, double* out, hipStream_t s);
+// BC, BX and BY planes from the corner plane DC (R rows)
+void launch_beds_from_corners(const Dev& d, int R, hipStream_t s);
 
 // Fused step (kp_step2.hip).  One wavefront marches one unit: 60 output
 // columns (lanes 2..61) by one 14-row tile row.

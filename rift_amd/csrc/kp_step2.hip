@@ -240,20 +240,28 @@ __global__ __launch_bounds__(256) void k_compact_quads(Dev d)
 // ---------------------------------------------------------------------------
 // marching kernels
 // ---------------------------------------------------------------------------
-struct Raw {                 // global loads of one row, issued a row ahead
-    float h, hu, hv, b, by;
+// Global loads of one row, issued a row ahead.  The beds come from the
+// corner plane: the loads of row j carry corner row j + 1 when marching down
+// and corner row j when marching up, the row a step further on, so that
+// finishing a row needs only corner rows already loaded (its own loads and
+// those of the row finished before it).  Each lane loads the corners of its
+// column and the next, whose sum is the row's pair sum.
+struct Raw {
+    float h, hu, hv;
+    float dc, dc1;               // corners of columns c and c + 1
     float dh, dhu, dhv, sg, n;   // stage 2
 };
 
 struct Row {
-    float h, hu, hv, b;     // state used by the flux (U^n in stage 1, U* in 2)
+    float h, hu, hv;        // state used by the flux (U^n in stage 1, U* in 2)
+    float b, by, bx;        // beds: cell, south face, west face
     float hraw;             // stage 2: U^n depth as stored (stage-1 sources)
     float hun, hvn;         // stage 2: U^n momenta of the cell (own-cell Heun)
     float d1;               // stage 2: the stage-1 dh of the cell (volume budget)
 };
 
 struct Src {                 // own-cell inputs of an output row
-    float bx, K, F, rain, drain;
+    float K, F, rain, drain;
 };
 
 // Per-lane constants of one unit.  The rows of a unit touch tile rows r-1,
@@ -366,7 +374,7 @@ __device__ __forceinline__ void surge_scalars(const Dev& d, StepScalars& S)
     }
 }
 
-template <int Stage, unsigned FM>
+template <int Stage, bool Up, unsigned FM>
 __device__ __forceinline__ Raw fetch(const Dev& d, const StepScalars& S, const Lane& L, int j)
 {
     const int sr = min(max(j, 2), d.ny - 3);
@@ -378,8 +386,9 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const StepScalars& S, const L
     r.h = ld(d.h + rb, lo);
     r.hu = ld(d.hu + rb, lo);
     r.hv = ld(d.hv + rb, lo);
-    r.b = ld(d.BC + rb, sc4);
-    r.by = ld(d.BY + row_of(d, j), c4);
+    const float* dcr = d.DC + row_of(d, Up ? j : j + 1);
+    r.dc = ld(dcr, c4);
+    r.dc1 = ld(dcr, c4 + 4u);
     if constexpr (Stage == 2) {
         r.dh = ld(d.dh + rb, sc4);
         r.dhu = ld(d.dhu + rb, sc4);
@@ -404,7 +413,6 @@ __device__ __forceinline__ Src fetch_src(const Dev& d, const Lane& L, int j)
     const size_t rb = row_of(d, j);
     const unsigned c4 = lane_off<Stage, FM>(L.c4);
     Src s;
-    s.bx = ld(d.BX + rb, c4);
     s.K = (d.flags & RIFT_INFILTRATION) ? ld(d.K + rb, c4) : 0.f;
     s.F = (d.flags & RIFT_INFILTRATION) ? ld(d.F + rb, c4) : 0.f;
     s.rain = (d.flags & FM & RIFT_RAIN_GRID) ? ld(d.map[RIFT_MAP_RAIN][0] + rb, c4) : 0.f;
@@ -412,15 +420,26 @@ __device__ __forceinline__ Src fetch_src(const Dev& d, const Lane& L, int j)
     return s;
 }
 
-template <int Stage, unsigned FM>
+// Row j from its loads; dcp, pp: corner and pair sum of the corner row the
+// row finished before it loaded (the other corner row of row j), replaced by
+// this row's.
+template <int Stage, bool Up, unsigned FM>
 __device__ __forceinline__ Row finish(const Dev& d, const Lane& L, const Raw& raw, int j, float dt,
-                                      float inv_dt)
+                                      float inv_dt, float& dcp, float& pp)
 {
     const int sr = min(max(j, 2), d.ny - 3);
     const bool gy = (j < 2) || (j > d.ny - 3);
     float h = raw.h, hu = raw.hu, hv = raw.hv;
     Row r;
-    r.b = raw.b;
+    {
+        const float pc = raw.dc + raw.dc1;
+        const float plo = Up ? pc : pp, phi = Up ? pp : pc;
+        r.b = bed_cell(plo, phi);
+        r.by = bed_south(plo);
+        r.bx = Up ? bed_west(raw.dc, dcp) : bed_west(dcp, raw.dc);
+        dcp = raw.dc;
+        pp = pc;
+    }
     if constexpr (Stage == 2) {
         r.hraw = h;
         r.hun = hu;
@@ -726,37 +745,44 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
     Src SR[3];         // own-cell inputs of the output row and the next
     FaceStates Y[3];   // y face states of C (and D once formed)
     Flux G[3];         // y flux of the face behind C (and between C and D)
-    float BY[3];       // BY of a row = the bed of its south face
 
-    // prologue (phase 0): rows j-2 .. j+2 (both directions)
-    const Row A = finish<Stage, FM>(d, L, fetch<Stage, FM>(d, S, L, j - 2 * step), j - 2 * step, dt, inv_dt);
-    const Raw rB = fetch<Stage, FM>(d, S, L, j - step);
-    const Raw rC = fetch<Stage, FM>(d, S, L, j);
-    const Raw rD = fetch<Stage, FM>(d, S, L, j + step);
-    W[2] = fetch<Stage, FM>(d, S, L, j + 2 * step);
+    // prologue (phase 0): rows j-2 .. j+2 (both directions).  The first
+    // finished row, A, also needs the corner row before its own loads' one
+    // (row j-2 down, j+3 up).
+    float dcp, pp;
+    {
+        const float* xr = d.DC + row_of(d, Up ? j + 3 : j - 2);
+        const unsigned c4 = lane_off<Stage, FM>(L.c4);
+        dcp = ld(xr, c4);
+        pp = dcp + ld(xr, c4 + 4u);
+    }
+    const Row A = finish<Stage, Up, FM>(d, L, fetch<Stage, Up, FM>(d, S, L, j - 2 * step), j - 2 * step, dt,
+                                        inv_dt, dcp, pp);
+    const Raw rB = fetch<Stage, Up, FM>(d, S, L, j - step);
+    const Raw rC = fetch<Stage, Up, FM>(d, S, L, j);
+    const Raw rD = fetch<Stage, Up, FM>(d, S, L, j + step);
+    W[2] = fetch<Stage, Up, FM>(d, S, L, j + 2 * step);
     SR[0] = fetch_src<Stage, FM>(d, L, j);
-    const Row B = finish<Stage, FM>(d, L, rB, j - step, dt, inv_dt);
-    R[0] = finish<Stage, FM>(d, L, rC, j, dt, inv_dt);
-    R[1] = finish<Stage, FM>(d, L, rD, j + step, dt, inv_dt);
-    BY[0] = rC.by;
-    BY[1] = rD.by;
-    BY[2] = rB.by;
+    const Row B = finish<Stage, Up, FM>(d, L, rB, j - step, dt, inv_dt, dcp, pp);
+    R[0] = finish<Stage, Up, FM>(d, L, rC, j, dt, inv_dt, dcp, pp);
+    R[1] = finish<Stage, Up, FM>(d, L, rD, j + step, dt, inv_dt, dcp, pp);
+    // the row behind C (its south face bed is read as by_behind below)
+    R[2].by = B.by;
     {
         const Row& C = R[0];
         const Row& D = R[1];
         if constexpr (!Up) {
             const FaceStates yB = construct<Textbook>(A.h, B.h, C.h, A.b, B.b, C.b, A.hu, B.hu, C.hu,
-                                                      A.hv, B.hv, C.hv, rC.by, rB.by);
+                                                      A.hv, B.hv, C.hv, C.by, B.by);
             Y[0] = construct<Textbook>(B.h, C.h, D.h, B.b, C.b, D.b, B.hu, C.hu, D.hu, B.hv, C.hv, D.hv,
-                                       rD.by, rC.by);
+                                       D.by, C.by);
             G[0] = face_flux<1>(yB.hp, yB.qxp, yB.qyp, Y[0].hm, Y[0].qxm, Y[0].qym, kappa);
         } else {
             // rows: A = j+2, B = j+1 (north of C), D = j-1
-            const float byA = ld(d.BY + row_of(d, j + 2), L.c4);
             const FaceStates yB = construct<Textbook>(C.h, B.h, A.h, C.b, B.b, A.b, C.hu, B.hu, A.hu,
-                                                      C.hv, B.hv, A.hv, byA, rB.by);
+                                                      C.hv, B.hv, A.hv, A.by, B.by);
             Y[0] = construct<Textbook>(D.h, C.h, B.h, D.b, C.b, B.b, D.hu, C.hu, B.hu, D.hv, C.hv, B.hv,
-                                       rB.by, rC.by);
+                                       B.by, C.by);
             G[0] = face_flux<1>(Y[0].hp, Y[0].qxp, Y[0].qyp, yB.hm, yB.qxm, yB.qym, kappa);
         }
     }
@@ -780,23 +806,22 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         // prefetch: the row after next and the next output row's inputs
         // (clamped to the domain; unused past the end of the run)
         const int jn = Up ? max(j - 3, 0) : j + 3;
-        const Raw rN = fetch<Stage, FM>(d, S, L, jn);
+        const Raw rN = fetch<Stage, Up, FM>(d, S, L, jn);
         SR[kd] = fetch_src<Stage, FM>(d, L, min(max(j + step, 2), d.ny - 3));
 
-        const float by_behind = BY[ke];    // BY of row j - step
-        R[ke] = finish<Stage, FM>(d, L, W[ke], j + 2 * step, dt, inv_dt);
-        BY[ke] = W[ke].by;
+        const float by_behind = R[ke].by;    // BY of row j - step
+        R[ke] = finish<Stage, Up, FM>(d, L, W[ke], j + 2 * step, dt, inv_dt, dcp, pp);
         W[kc] = rN;
         const Row& C = R[kc];
         const Row& D = R[kd];
         const Row& E = R[ke];
         if constexpr (!Up) {
             Y[kd] = construct<Textbook>(C.h, D.h, E.h, C.b, D.b, E.b, C.hu, D.hu, E.hu, C.hv, D.hv, E.hv,
-                                        BY[ke], BY[kd]);
+                                        E.by, D.by);
             G[kd] = face_flux<1>(Y[kc].hp, Y[kc].qxp, Y[kc].qyp, Y[kd].hm, Y[kd].qxm, Y[kd].qym, kappa);
         } else {
             Y[kd] = construct<Textbook>(E.h, D.h, C.h, E.b, D.b, C.b, E.hu, D.hu, C.hu, E.hv, D.hv, C.hv,
-                                        BY[kc], BY[kd]);
+                                        C.by, D.by);
             G[kd] = face_flux<1>(Y[kd].hp, Y[kd].qxp, Y[kd].qyp, Y[kc].hm, Y[kc].qxm, Y[kc].qym, kappa);
         }
         // north / south faces of row j
@@ -805,7 +830,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         const Src& src = SR[kc];
 
         // x pass of row j
-        const float bxw = src.bx;
+        const float bxw = C.bx;
         const float bxe = from_right(bxw);
         const FaceStates xs = construct<Textbook>(
             from_left(C.h), C.h, from_right(C.h), from_left(C.b), C.b, from_right(C.b),
@@ -822,7 +847,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             mxl = fmaxf(mxl, fE.speed);
             myl = fmaxf(myl, gn.speed);
             // beds of the south / north faces of row j
-            const float bys = BY[kc], byn = Up ? by_behind : BY[kd];
+            const float bys = C.by, byn = Up ? by_behind : D.by;
             wetl |= output_row<Stage, Budget, FM>(d, S, L, c, j, C, src, dh, dhu, dhv, bxw, bxe, bys, byn,
                                                   fE.f0, fw0, gn.f0, gs.f0, acc);
         }
@@ -840,16 +865,14 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             // one phase, the window advanced by copies
             for (;;) {
                 if (row(Phase<0>{})) break;
+                const float by_c = R[0].by;   // the row behind the next output row
                 R[0] = R[1];
                 R[1] = R[2];
+                R[2].by = by_c;
                 W[2] = W[0];
                 SR[0] = SR[1];
                 Y[0] = Y[1];
                 G[0] = G[1];
-                const float by0 = BY[0];
-                BY[0] = BY[1];
-                BY[1] = BY[2];
-                BY[2] = by0;
             }
         }
     }

@@ -14,6 +14,7 @@
 #include <utility>
 #include <vector>
 
+#include "../../include/rift_host.h"
 #include "../../include/rift_kp.h"
 #include "kp_device.hpp"
 #include "kp_kernels.hpp"
@@ -76,6 +77,9 @@ struct rift_grid {
     int cus = 0;                  // compute units of the device
     int march_blocks[2][kMarchVariants] = {};   // resident workgroups per stage/variant
     bool per_phase = false;       // rift_step uses the one-kernel-per-phase path
+    // the face beds are means of one corner field (d.DC holds it): the
+    // fused step, which forms the beds from the corners, is available
+    bool corner_beds = false;
     // captured fused steps (1 and kGraphSteps steps); the kernels take the
     // handle's Dev by value, so any change of it invalidates the graphs
     hipGraphExec_t graph1 = nullptr, graphN = nullptr;
@@ -365,6 +369,10 @@ int rift_create(const rift_desc* desc, rift_grid** out)
         }
     }
     P(d.dh); P(d.dhu); P(d.dhv); P(d.G); P(d.n); P(d.BC); P(d.BX); P(d.BY);
+    // corners: one row more than a plane's padding needs is never read;
+    // the marching kernels' last lane reads one column past its own
+    d.DC = alloc_bytes<float>(g, g->plane + (size_t)d.P);
+    ok = ok && d.DC;
     if (f & RIFT_INFILTRATION) { P(d.F); P(d.F0); P(d.dF); P(d.K); }
     if (f & RIFT_SAVE_MAX) { P(d.hmax); P(d.qmax); P(d.tpeak); P(d.tdry); }
     if (f & RIFT_SAVE_ARRIVAL) P(d.twet);
@@ -423,6 +431,31 @@ int rift_destroy(rift_grid* g)
     return RIFT_OK;
 }
 
+// The corner plane in the field layout, extended with period 2 beyond the
+// corners (D(1) = D(3), D(0) = D(2), D(nx-1) = D(nx-3), D(nx) = D(nx-2), and
+// so on; rows alike).  Formed from it, every ghost face is the face two
+// cells in, and every ghost cell's bed is that of the interior cell its
+// state comes from, as in the reference's arrays.
+static int put_corners(rift_grid* g, const std::vector<double>& D)
+{
+    const Dev& d = g->d;
+    const int W = d.nx - 3, H = d.ny - 3;
+    auto ext = [](int c, int n) {
+        if (c < 0) return c & 1;
+        if (c > n - 1) return n - 1 - ((n - 1 - c) & 1);
+        return c;
+    };
+    std::vector<float> tmp(g->plane + (size_t)d.P, 0.f);
+    for (int j = 0; j < g->R; ++j) {
+        const int cj = ext(j - 2, H);
+        for (int i = 0; i < d.P; ++i) tmp[(size_t)j * d.P + i] = (float)D[(size_t)cj * W + ext(i - 2, W)];
+    }
+    HIPCHK(hipMemcpyAsync(d.DC, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, g->stream));
+    launch_beds_from_corners(d, g->R, g->stream);
+    HIPCHK(hipStreamSynchronize(g->stream));
+    return check_launch("beds from corners");
+}
+
 int rift_upload_static(rift_grid* g, const double* BX, const double* BY, const double* n,
                        const double* K, const double* h0)
 {
@@ -452,6 +485,15 @@ int rift_upload_static(rift_grid* g, const double* BX, const double* BY, const d
         }
     }
     if ((rc = put_field(g, d.BC, bc.data(), d.P, g->R, d.P))) return rc;
+    // beds that are means of one corner field (the reference's geometry
+    // always gives them) are formed on the device from the corners, as the
+    // fused step forms them; any other beds keep the uploaded planes and run
+    // the per-phase path
+    {
+        std::vector<double> corners((size_t)(nx - 3) * (ny - 3));
+        g->corner_beds = rift_corners_from_faces(nx, ny, BX, BY, corners.data()) == RIFT_OK;
+        if (g->corner_beds && (rc = put_corners(g, corners))) return rc;
+    }
     std::vector<double> zero((size_t)nx * ny, 0.0);
     g->n_host.assign(n ? n : zero.data(), (n ? n : zero.data()) + (size_t)nx * ny);
     if ((rc = put_field(g, d.n, g->n_host.data(), nx, ny, nx))) return rc;
@@ -696,6 +738,14 @@ constexpr int kGraphSteps = 16;
 
 static int step_fused(rift_grid* g);
 
+// rift_step runs the fused step unless asked for the per-phase path, in
+// Euler mode (the fused step is Heun's), or when the face beds are not means
+// of one corner field (the fused step forms every bed from the corners).
+static bool fused_path(const rift_grid* g)
+{
+    return !g->per_phase && !(g->d.flags & RIFT_EULER) && g->corner_beds;
+}
+
 static int capture_steps(rift_grid* g, int n, hipGraphExec_t* out)
 {
     hipGraph_t graph = nullptr;
@@ -759,8 +809,7 @@ static int step_fused(rift_grid* g)
 // replays graphs, whatever its step count.
 static int prepare_graphs(rift_grid* g)
 {
-    if (!g->inited || g->prof || !g->use_graphs || g->per_phase || (g->d.flags & RIFT_EULER))
-        return RIFT_OK;
+    if (!g->inited || g->prof || !g->use_graphs || !fused_path(g)) return RIFT_OK;
     prime(g);
     int rc;
     if (!g->graphN && (rc = capture_steps(g, kGraphSteps, &g->graphN))) return rc;
@@ -774,7 +823,7 @@ int rift_step(rift_grid* g, int nsteps)
     if (!g->inited) return fail(RIFT_E_STATE, "rift_init not called");
     if (nsteps < 0) return fail(RIFT_E_ARG, "rift_step: negative step count");
     const bool euler = g->d.flags & RIFT_EULER;
-    if (!euler && !g->per_phase) {
+    if (fused_path(g)) {
         if (g->prof || !g->use_graphs) {
             for (int s = 0; s < nsteps; ++s) {
                 int rc = step_fused(g);
@@ -890,6 +939,7 @@ static int copy_state(rift_grid* g, rift_grid* c)
     c->bc_ref = g->bc_ref;
     c->n_host = g->n_host;
     c->static_loaded = g->static_loaded;
+    c->corner_beds = g->corner_beds;
     c->inited = g->inited;
     c->scattered = g->scattered;
     c->primed = g->primed;
@@ -900,6 +950,8 @@ int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
 {
     if (!g || !best || nsteps < 1) return fail(RIFT_E_ARG, "rift_autotune: bad argument");
     if (!g->inited) return fail(RIFT_E_STATE, "rift_init not called");
+    if (!g->corner_beds)
+        return fail(RIFT_E_UNSUPPORTED, "rift_autotune: the face beds are not means of one corner field (no fused step)");
     rift_grid* c = nullptr;
     int rc = rift_create(&g->desc, &c);
     if (rc) return rc;
@@ -1566,6 +1618,9 @@ int rift_ensemble_step(rift_ensemble* e, int nsteps)
         rift_grid* g = e->members[i];
         if (!g->inited) return fail(RIFT_E_STATE, "rift_ensemble_step: member not initialised");
         if (g->per_phase) return fail(RIFT_E_UNSUPPORTED, "rift_ensemble_step: member on the per-phase path");
+        if (!g->corner_beds)
+            return fail(RIFT_E_UNSUPPORTED,
+                        "rift_ensemble_step: member's face beds are not means of one corner field");
         prime(g);
         e->host_devs[i] = g->d;
     }

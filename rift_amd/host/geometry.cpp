@@ -57,6 +57,44 @@ extern "C" int rift_geometry(int b_nx, int b_ny, const double* dem, const double
     return RIFT_OK;
 }
 
+// Corners from the face means: walked from D(0, 0) = 0 along the first row
+// (BY: D(i+1) = 2 BY - D(i)) and up every column (BX), then the
+// checkerboard's mean is taken out (a DEM has almost none, so the corners
+// come out close to its points and round to FP32 well).  Every face is then
+// checked against the corners' mean, and the ghost faces against the faces
+// two cells in (simulator.cu:404-414).
+extern "C" int rift_corners_from_faces(int nx, int ny, const double* BX, const double* BY, double* D)
+{
+    if (nx < 6 || ny < 6 || !BX || !BY || !D) return RIFT_E_ARG;
+    const int W = nx - 3, H = ny - 3;   // corner columns / rows
+    std::vector<double> c((size_t)W * H, 0.0);
+    auto bx = [&](int i, int j) { return BX[(size_t)j * (nx + 1) + i]; };
+    auto by = [&](int i, int j) { return BY[(size_t)j * nx + i]; };
+    auto at = [&](int ci, int cj) -> double& { return c[(size_t)cj * W + ci]; };
+    for (int ci = 0; ci + 1 < W; ++ci) at(ci + 1, 0) = 2.0 * by(ci + 2, 2) - at(ci, 0);
+    for (int cj = 0; cj + 1 < H; ++cj)
+        for (int ci = 0; ci < W; ++ci) at(ci, cj + 1) = 2.0 * bx(ci + 2, cj + 2) - at(ci, cj);
+    double cb = 0.0;
+    for (int cj = 0; cj < H; ++cj)
+        for (int ci = 0; ci < W; ++ci) cb += ((ci + cj) & 1) ? -at(ci, cj) : at(ci, cj);
+    cb /= (double)W * (double)H;
+    for (int cj = 0; cj < H; ++cj)
+        for (int ci = 0; ci < W; ++ci) at(ci, cj) -= ((ci + cj) & 1) ? -cb : cb;
+    auto near = [](double a, double b) { return std::fabs(a - b) <= 1e-9 * (1.0 + std::fabs(b)); };
+    bool ok = true;
+    for (int j = 2; j <= ny - 3 && ok; ++j) {
+        for (int i = 2; i <= nx - 2 && ok; ++i) ok = near(0.5 * (at(i - 2, j - 2) + at(i - 2, j - 1)), bx(i, j));
+        ok = ok && bx(1, j) == bx(3, j) && bx(0, j) == bx(2, j) && bx(nx - 1, j) == bx(nx - 3, j);
+    }
+    for (int j = 2; j <= ny - 2 && ok; ++j)
+        for (int i = 2; i <= nx - 3 && ok; ++i) ok = near(0.5 * (at(i - 2, j - 2) + at(i - 1, j - 2)), by(i, j));
+    for (int i = 2; i <= nx - 3 && ok; ++i)
+        ok = by(i, 1) == by(i, 3) && by(i, 0) == by(i, 2) && by(i, ny - 1) == by(i, ny - 3);
+    if (!ok) return RIFT_E_ARG;
+    std::memcpy(D, c.data(), c.size() * sizeof(double));
+    return RIFT_OK;
+}
+
 // GridSeries::p_interp (GridSeries.cu:91-122): a cell takes the mean of its
 // four corner values when more than two of them hold data, times the series
 // scale; every other cell (ghosts included) holds the fill value, nodata

@@ -110,6 +110,7 @@ SIGNATURES = [
     ("rift_wet_cell_updates", C.c_int, [_VP, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     ("rift_kappa", C.c_double, [C.c_double, C.c_double]),
     ("rift_geometry", C.c_int, [C.c_int, C.c_int, _D, _D, _D, C.c_double, _D, _D, _D, _D, _D, _D]),
+    ("rift_corners_from_faces", C.c_int, [C.c_int, C.c_int, _D, _D, _D]),
     ("rift_read_series", C.c_int, [C.c_char_p, _F, _F, C.c_int, _I]),
     ("rift_read_raster", C.c_int, [C.c_char_p, C.c_int, _I, _I, _D, _D, C.c_long]),
     ("rift_write_grid", C.c_int, [C.c_char_p, _D, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]),
@@ -207,6 +208,18 @@ def geometry(dem, h0=None, n=None, n_const=0.0, K=None):
                                 _dp(arrs[2]), _dp(BX), _dp(BY), _dp(H0), _dp(N), _dp(KK)),
            "rift_geometry")
     return {"BX": BX, "BY": BY, "h0": H0, "n": N, "K": KK, "nx": nx, "ny": ny}
+
+
+def corners_from_faces(BX, BY):
+    """Corner elevations (ny-3, nx-3) behind face beds BX (ny, nx+1) and BY
+    (ny+1, nx); None when the faces are not means of one corner field."""
+    BX = np.ascontiguousarray(BX, dtype=np.float64)
+    BY = np.ascontiguousarray(BY, dtype=np.float64)
+    ny, nx1 = BX.shape
+    nx = nx1 - 1
+    D = np.empty((ny - 3, nx - 3))
+    rc = load().rift_corners_from_faces(nx, ny, _dp(BX), _dp(BY), _dp(D))
+    return D if rc == 0 else None
 
 
 class DeviceArray:

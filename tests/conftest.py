@@ -35,3 +35,13 @@ def rift():
     from rift_amd import kp
     kp.load()
     return kp
+
+
+@pytest.fixture
+def progress(capsys):
+    """Progress lines of a long test, written past pytest's output capture
+    so that a run that watches its output sees the test working."""
+    def say(msg):
+        with capsys.disabled():
+            print(msg, flush=True)
+    return say

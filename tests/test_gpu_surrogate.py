@@ -96,16 +96,17 @@ def test_blown_up_member_is_ended_and_reported(S):
     assert np.isnan(Y[0]).all() and np.isfinite(Y[1]).all()
 
 
-def test_full_size_heldout_error_within_5_cm(S):
+def test_full_size_heldout_error_within_5_cm(S, progress):
     """The full surrogate workload (4000 training runs, 2048 test scenarios
     at 256 x 256, 20 epochs): on the 5 % of test scenarios held out and
     re-run by the solver, the network's maximum-depth map is within 5 cm of
     the solver's (RMSE over the cells, and the 95th percentile of the
     per-scenario mean error).  Scenario inputs come from spawned workers
     (this process already uses the GPU)."""
+    progress("surrogate: generating 4000 + 2048 scenarios")
     data = S.prepare(4000, 2048, workers=8, start="spawn")
-    print("scenarios generated", flush=True)
-    r = S.run(data=data, epochs=20, solve_all=False, log=lambda m: print("surrogate:", m, flush=True))
+    progress("surrogate: scenarios generated")
+    r = S.run(data=data, epochs=20, solve_all=False, log=lambda m: progress(f"surrogate: {m}"))
     e = r["heldout_errors"]
     print(f"held out {r['heldout']}: max-depth MAE {e['depth_mae_m']:.4f} m, RMSE {e['depth_rmse_m']:.4f} m, "
           f"p95 {e['depth_mae_p95_m']:.4f} m; fallback {r['fallback']}", flush=True)
