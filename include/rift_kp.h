@@ -193,11 +193,12 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
  * 0 .. RIFT_NVARIANTS-1 (default 0): bit 0 = stage 1 with a larger register
  * budget (4 instead of 5 waves per SIMD), bit 1 = stage 2 likewise (3
  * instead of 4), bit 2 = work shares dealt to workgroups in launch order
- * instead of by XCD, bit 3 = shares of at least 7 rows instead of 14 (more
- * waves on sparse steps).  Every variant computes the same results bit for
- * bit. */
+ * instead of by XCD, bits 3-4 = least rows a wave marches in one share (the
+ * cells per lane of a share: 14, 7, 28 or 42), bit 5 = every active unit
+ * marched by a wave of its own (no workgroups of four neighbouring strips).
+ * Every variant computes the same results bit for bit. */
 #define RIFT_OPT_VARIANT 3
-#define RIFT_NVARIANTS 16
+#define RIFT_NVARIANTS 64
 /* RIFT_OPT_MASS_BUDGET: 1 = step with the FP64 volume budget kernels (as
  * RIFT_MASS_BUDGET at creation), 0 = without; switching it on resets the
  * budget (rift_budget_reset).  The state arithmetic is the same either way. */

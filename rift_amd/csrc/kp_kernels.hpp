@@ -121,6 +121,7 @@ struct Dev {
     double tf;                   // end time of an ensemble member (0: none)
     int band;                    // XCD-banded order of the marching shares
     int min_rows;                // least rows per marching share
+    int quads;                   // single domain: full quads marched by one workgroup
     float *mx, *my;              // per-tile maxima of the last flux pass
     float *mx1, *my1;            // fused step: stage-1 maxima of this step
     Scalars* sc;

@@ -203,7 +203,7 @@ __device__ __forceinline__ void compact_quads_body(const Dev& d, int u)
             const int tag = d.sc->ustep + 1;
             own = d.utag[u] == tag;
             const int s0 = s & ~3;
-            if (own && s0 + 3 < d.nstrips) {
+            if (own && d.quads && s0 + 3 < d.nstrips) {
                 full = true;
                 for (int k = s0; k < s0 + 4; ++k) full = full && d.utag[k * d.nurows + r] == tag;
             }

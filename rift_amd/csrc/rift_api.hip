@@ -336,6 +336,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.nsrc = 0;
     d.band = 1;
     d.min_rows = kTileH;
+    d.quads = 1;
 
     if (g_member_stream) {
         g->stream = g_member_stream;
@@ -910,7 +911,11 @@ int rift_set_option(rift_grid* g, int option, int value)
         drop_graphs(g);   // the graphs hold the launch configuration
         g->variant = value & (kMarchVariants - 1);
         g->d.band = (value & kMarchVariants) ? 0 : 1;
-        g->d.min_rows = (value & (2 * kMarchVariants)) ? kTileH / 2 : kTileH;
+        {
+            static const int least_rows[4] = {kTileH, kTileH / 2, 2 * kTileH, 3 * kTileH};
+            g->d.min_rows = least_rows[(value >> 3) & 3];
+        }
+        g->d.quads = (value & 32) ? 0 : 1;
         return prepare_graphs(g);
     default:
         return fail(RIFT_E_ARG, "rift_set_option: unknown option");

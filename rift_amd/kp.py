@@ -37,7 +37,7 @@ SER_HYETO, SER_DRAIN, SER_HYDRO = 0, 1, 2
 MAP_RAIN, MAP_DRAIN, MAP_SURGE = 0, 1, 2
 MASS_BUDGET = 1 << 13
 OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT, OPT_MASS_BUDGET = 1, 2, 3, 4
-NVARIANTS = 16
+NVARIANTS = 64
 KERNEL_CLASSES = ["flux1", "stage2_flux", "integrate", "boundaries", "grow", "timestep"]
 
 # (name, restype, argtypes) of every exported entry point
