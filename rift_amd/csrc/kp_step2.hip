@@ -163,6 +163,40 @@ __device__ __forceinline__ void unit_surge_flags(const Dev& d, int u, int s, int
     d.sgu[u] = rel ? 1 : 0;
 }
 
+// List appends of a workgroup of 256: for each of N lists, the flagged
+// threads get consecutive slots (waves in order, lanes in order within a
+// wave) from one atomic on the list's counter per workgroup, not one per
+// wave (same-address atomics from ~1300 waves serialise); slot[k] is this
+// thread's slot in list k (valid where flag[k]).  Every thread of the
+// workgroup must call it.
+template <int N>
+__device__ __forceinline__ void wg_append(const bool (&flag)[N], int* const (&counter)[N], int (&slot)[N])
+{
+    __shared__ int s_cnt[4][N];
+    __shared__ int s_base[N];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long m[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        m[k] = __ballot(flag[k]);
+        if (lane == 0) s_cnt[wave][k] = __popcll(m[k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        const int k = threadIdx.x;
+        const int tot = s_cnt[0][k] + s_cnt[1][k] + s_cnt[2][k] + s_cnt[3][k];
+        s_base[k] = tot ? atomicAdd(counter[k], tot) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int b = s_base[k];
+        for (int w = 0; w < wave; ++w) b += s_cnt[w][k];
+        slot[k] = b + __popcll(m[k] & lt);
+    }
+}
+
 __device__ __forceinline__ void compact_units_body(const Dev& d, int u)
 {
     const bool is_unit = u < d.nunits;
@@ -176,13 +210,11 @@ __device__ __forceinline__ void compact_units_body(const Dev& d, int u)
             unit_surge_flags(d, u, s, u - s * d.nurows);
         }
     }
-    const int lane = threadIdx.x & 63;
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    const unsigned long long mu = __ballot(ua);
-    int ubase = 0;
-    if (lane == 0 && mu) ubase = atomicAdd(&d.sc->n_units_acc, __popcll(mu));
-    ubase = __shfl(ubase, 0);
-    if (ua) d.ulist[ubase + __popcll(mu & lt)] = u;
+    int* const ctr[1] = {&d.sc->n_units_acc};
+    const bool f[1] = {ua};
+    int slot[1];
+    wg_append<1>(f, ctr, slot);
+    if (ua) d.ulist[slot[0]] = u;
 }
 
 // Single domain, one thread per unit u = (s, r), strip-major, after every
@@ -212,19 +244,12 @@ __device__ __forceinline__ void compact_quads_body(const Dev& d, int u)
     }
     const bool part = own && !full;
     const bool head = full && (s & 3) == 0;
-    const int lane = threadIdx.x & 63;
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    const unsigned long long mp = __ballot(part), mh = __ballot(head), mo = __ballot(own);
-    int pbase = 0, hbase = 0;
-    if (lane == 0) {
-        if (mo) atomicAdd(&d.sc->n_units_acc, __popcll(mo));
-        if (mp) pbase = atomicAdd(&d.sc->n_part_acc, __popcll(mp));
-        if (mh) hbase = atomicAdd(&d.sc->n_quads_acc, __popcll(mh));
-    }
-    pbase = __shfl(pbase, 0);
-    hbase = __shfl(hbase, 0);
-    if (part) d.ulist[pbase + __popcll(mp & lt)] = u;
-    if (head) d.qlist[hbase + __popcll(mh & lt)] = (s >> 2) * d.nurows + r;
+    int* const ctr[3] = {&d.sc->n_units_acc, &d.sc->n_part_acc, &d.sc->n_quads_acc};
+    const bool f[3] = {own, part, head};
+    int slot[3];
+    wg_append<3>(f, ctr, slot);
+    if (part) d.ulist[slot[1]] = u;
+    if (head) d.qlist[slot[2]] = (s >> 2) * d.nurows + r;
 }
 
 __global__ __launch_bounds__(256) void k_grow_tiles(Dev d)
@@ -1109,6 +1134,17 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
     __shared__ int s_match[3];
     DtRed r{0.f, 0.f, 0ull, 0};
     const int tid = threadIdx.x;
+    Scalars* sc = d.sc;
+    // the tail runs on a copy of the device scalars in LDS, staged by every
+    // workgroup up front (in flight with its tile loads; only the last one
+    // uses it) and written back by the last one: no round trip in the tail.
+    // Nothing changes the scalars while this kernel runs but the ticket,
+    // which the tail clears.
+    __shared__ Scalars ssc;
+    constexpr int kWords = (int)(sizeof(Scalars) / sizeof(unsigned));
+    static_assert(sizeof(Scalars) % sizeof(unsigned) == 0, "Scalars is copied in 32-bit words");
+    for (int k = tid; k < kWords; k += 256)
+        reinterpret_cast<unsigned*>(&ssc)[k] = reinterpret_cast<const unsigned*>(sc)[k];
     static_assert(kDtTilesPerBlock == 4 * 256, "four consecutive tiles per lane");
     // four consecutive tiles per lane: one 32-bit load of the activity bytes
     // and one 128-bit load of each maxima array, all in flight together (the
@@ -1142,7 +1178,6 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
         }
     }
     r = dt_block_reduce(r, red4);
-    Scalars* sc = d.sc;
     if (tid == 0) {
         DtPartial* p = d.dtp + blockIdx.x;
         __hip_atomic_store(&p->vx, r.vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1166,14 +1201,6 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
     __syncthreads();   // red4 is reused
     r = dt_block_reduce(r, red4);
     if (tid == 0) total = r;
-    // the tail runs on a copy of the device scalars in LDS, staged and
-    // written back by the whole workgroup (one round trip each way instead
-    // of a chain of dependent single-lane accesses)
-    __shared__ Scalars ssc;
-    constexpr int kWords = (int)(sizeof(Scalars) / sizeof(unsigned));
-    static_assert(sizeof(Scalars) % sizeof(unsigned) == 0, "Scalars is copied in 32-bit words");
-    for (int k = tid; k < kWords; k += 256)
-        reinterpret_cast<unsigned*>(&ssc)[k] = reinterpret_cast<const unsigned*>(sc)[k];
     __syncthreads();
     if (tid == 0) {
         ssc.ticket = 0u;

@@ -320,4 +320,12 @@ def test_device_volume_budget(rift, accel):
         assert abs(bg[k] - bo[k]) <= tol[name] * max(abs(bo[k]), 1e-9), (name, bg[k], bo[k])
     # the exchange (dry clamps) is small here in both
     assert abs(bg[7] - bo[7]) <= 1e-3 * scale
+    # rift_mass_balance (the reference's host sum of h and F, simulator.cu:
+    # 609-637, as FP64 device sums): stored and infiltrated volume, and the
+    # rain and source volume added, against the oracle
+    mb = g.mass_balance()
+    assert mb[0] == bg[0] and mb[1] == bg[1]
+    for k, name in ((0, "stored"), (1, "infiltrated"), (2, "rain"), (3, "source")):
+        ref = bo[{0: 0, 1: 1, 2: 3, 3: 4}[k]]
+        assert abs(mb[k] - ref) <= 1e-4 * max(abs(ref), 1e-9), (name, mb[k], ref)
     g.close()
