@@ -476,6 +476,8 @@ def rain_basin_leg(args):
     cell wet, so the active set fills from tile 0 within a few hundred
     steps): K timed steps at full activity."""
     g, cfg = build_rain_basin(2048)
+    if not args.graphs:
+        g.set_graphs(False)
     steps = 0
     while True:
         g.step(50)
@@ -485,12 +487,16 @@ def rain_basin_leg(args):
             break
     log(f"rain basin at full activity after {steps} steps ({n}/{g.ntiles} tiles)")
     sb = STAGE_BYTES["rain_basin"]
-    w = timed_windows(g, max(args.steps, 50), 5, -1, sb)
+    w = timed_windows(g, max(args.steps, 50), 5, args.rain_variant, sb)
     g.close()
+    state = (f"rain_basin 2048 spin={steps} active={n} settle={SETTLE_STEPS} warmup=5 "
+             f"t_timed={w['t_start']:.4f}")
+    traffic = measured_traffic(RAIN_PROFILE, "k_march<2>" if w["dom"] == "march2" else "k_march<1>",
+                               state, w["variant"])
     return {"value": w["value"], "unit": "WCU/s", "ms_per_step": w["ms_per_step"],
             "workload": "rain_basin 2048x2048 dx=10m, rain + Green-Ampt + Manning, every tile active",
-            "spinup_steps": steps, "active_tiles": w["n_act"],
-            "roofline": roofline(w, sb, None),
+            "spinup_steps": steps, "active_tiles": w["n_act"], "profile_state": state,
+            "roofline": roofline(w, sb, traffic),
             "step_roofline": {"bytes_per_wcu": sb["step"], "frac": w["value"] * sb["step"] / HBM_PEAK},
             "kernel_ms_per_step": w["kernel_ms_per_step"], "launch_variant": w["variant"],
             "variant_ms_per_step": w["variant_ms"]}
@@ -594,6 +600,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1,
                     help="launch variant of the marching kernels (default: autotuned after the "
                          "spin-up, outside the timed region)")
+    ap.add_argument("--rain-variant", type=int, default=-1,
+                    help="launch variant of the rain-basin leg (default: autotuned)")
     ap.add_argument("--surrogate-train", type=int, default=4000,
                     help="training scenarios of the surrogate-screening leg (0: skip the leg)")
     ap.add_argument("--surrogate-test", type=int, default=2048,

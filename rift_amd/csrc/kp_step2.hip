@@ -36,14 +36,16 @@
 
 namespace rift {
 
-// lane L receives lane L-1's value (wave_shr:1) / lane L+1's (wave_shl:1)
+// lane L receives lane L-1's value (wave_shr:1) / lane L+1's (wave_shl:1);
+// lane 0 (63) receives 0 (bound_ctrl: the missing source reads as zero, so
+// no destination has to be zeroed first and the move can fold into its user)
 __device__ __forceinline__ float from_left(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_right(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float wave_max(float v)
@@ -304,9 +306,13 @@ struct Lane {
     unsigned pl4;            // bytes of one plane (second state buffer)
     int txo, r;
     bool gx;
-    bool sel0, sel1, sel2;   // buffer selects of tile rows r-1, r, r+1
+    // byte offsets of the lane's state cell in tile rows r-1, r, r+1: the
+    // clamped column, plus one plane where the tile's state lives in the
+    // second buffer (formed at tile-row crossings, not per row)
+    unsigned so0, so1, so2;
     bool act0, act1, act2;   // activity of tile rows r-1, r, r+1
-    bool selN, actN;         // the same for tile row r+2 (down) or r-2 (up)
+    unsigned soN;            // the same for tile row r+2 (down) or r-2 (up)
+    bool actN;
     // surge series: the strip's tiles of tile rows r-1, r, r+1 (and the next
     // one) can hold surge > 0 in this step's window (wave-uniform)
     bool sg0, sg1, sg2, sgN;
@@ -406,7 +412,7 @@ __device__ __forceinline__ Raw fetch(const Dev& d, const StepScalars& S, const L
     const int slot = tile_row_slot(L, sr);
     const size_t rb = row_of(d, sr);
     const unsigned sc4 = lane_off<Stage, FM>(L.sc4), c4 = lane_off<Stage, FM>(L.c4);
-    const unsigned lo = sc4 + (pick(slot, L.sel0, L.sel1, L.sel2) ? L.pl4 : 0u);
+    const unsigned lo = lane_off<Stage, FM>(pick(slot, L.so0, L.so1, L.so2));
     Raw r;
     r.h = ld(d.h + rb, lo);
     r.hu = ld(d.hu + rb, lo);
@@ -527,10 +533,12 @@ __device__ __forceinline__ Lane make_lane(const Dev& d, int s, int rr, int lane)
     L.r = rr;
     const int Tm = max(rr - 1, 0) * d.nbx + L.txo, T0 = rr * d.nbx + L.txo;
     const int Tp = min(rr + 1, d.nby - 1) * d.nbx + L.txo;
-    L.sel0 = d.sel[Tm] != 0; L.sel1 = d.sel[T0] != 0; L.sel2 = d.sel[Tp] != 0;
+    L.so0 = L.sc4 + (d.sel[Tm] ? L.pl4 : 0u);
+    L.so1 = L.sc4 + (d.sel[T0] ? L.pl4 : 0u);
+    L.so2 = L.sc4 + (d.sel[Tp] ? L.pl4 : 0u);
     L.act0 = d.act[Tm] != 0; L.act1 = d.act[T0] != 0; L.act2 = d.act[Tp] != 0;
     const int TN = (Up ? max(rr - 2, 0) : min(rr + 2, d.nby - 1)) * d.nbx + L.txo;
-    L.selN = d.sel[TN] != 0;
+    L.soN = L.sc4 + (d.sel[TN] ? L.pl4 : 0u);
     L.actN = d.act[TN] != 0;
     L.sg0 = unit_surge<FM>(d, s, rr - 1);
     L.sg1 = unit_surge<FM>(d, s, rr);
@@ -545,16 +553,16 @@ __device__ __forceinline__ void shift_lane(const Dev& d, Lane& L, int s, int rr)
 {
     L.r = rr;
     if constexpr (!Up) {
-        L.sel0 = L.sel1; L.sel1 = L.sel2; L.sel2 = L.selN;
+        L.so0 = L.so1; L.so1 = L.so2; L.so2 = L.soN;
         L.act0 = L.act1; L.act1 = L.act2; L.act2 = L.actN;
         L.sg0 = L.sg1; L.sg1 = L.sg2; L.sg2 = L.sgN;
     } else {
-        L.sel2 = L.sel1; L.sel1 = L.sel0; L.sel0 = L.selN;
+        L.so2 = L.so1; L.so1 = L.so0; L.so0 = L.soN;
         L.act2 = L.act1; L.act1 = L.act0; L.act0 = L.actN;
         L.sg2 = L.sg1; L.sg1 = L.sg0; L.sg0 = L.sgN;
     }
     const int TN = (Up ? max(rr - 2, 0) : min(rr + 2, d.nby - 1)) * d.nbx + L.txo;
-    L.selN = d.sel[TN] != 0;
+    L.soN = L.sc4 + (d.sel[TN] ? L.pl4 : 0u);
     L.actN = d.act[TN] != 0;
     L.sgN = unit_surge<FM>(d, s, Up ? rr - 2 : rr + 2);
 }
@@ -697,8 +705,9 @@ __device__ __forceinline__ bool output_row(const Dev& d, const StepScalars& S, c
             // by U*, clamp of U^{n+1}
             acc.exch += (h_n - C.hraw) + 0.5f * (C.h - fmaf(C.d1, S.dt, h_n)) + (hc - hn);
         }
-        // into the buffer the tile does not read
-        const unsigned lo = c4 + (L.sel1 ? 0u : L.pl4);
+        // into the buffer the tile does not read (an output lane's column
+        // is its clamped column: c4 + pl4 - (so1 - c4))
+        const unsigned lo = 2u * c4 + L.pl4 - L.so1;
         at(d.h + rbo, lo) = hc;
         at(d.hu + rbo, lo) = hun;
         at(d.hv + rbo, lo) = hvn;
