@@ -63,6 +63,44 @@ struct FaceStates {
     float hm, qxm, qym;   // west / south side
 };
 
+// Differences across the face between a cell and its neighbour in the pass
+// direction (n = plus side, s = minus side): of w = h + b, formed from depth
+// and bed differences, and of the momenta.  A face's difference is the plus
+// difference of the cell on its minus side and the minus difference of the
+// cell on its plus side, so the marching kernels form it once.
+struct FaceDiff {
+    float w, qx, qy;
+};
+__device__ __forceinline__ FaceDiff face_diff(float hs, float hn, float bs, float bn, float qxs, float qxn,
+                                              float qys, float qyn)
+{
+    return FaceDiff{(hn - hs) + (bn - bs), qxn - qxs, qyn - qys};
+}
+
+// Both face states of one cell from its cell values and the differences
+// across its plus (dp) and minus (dm) faces.
+template <bool Textbook>
+__device__ __forceinline__ FaceStates construct_d(float h, float b, float qx, float qy, const FaceDiff& dp,
+                                                  const FaceDiff& dm, float Bp, float Bm)
+{
+    FaceStates s;
+    float sw = half_slope<Textbook>(dp.w, dm.w);
+    const float hp0 = (h + (b - Bp)) + sw;
+    const float hm0 = (h + (b - Bm)) - sw;
+    const float sum = hp0 + hm0;
+    const float hp = fminf(__builtin_amdgcn_fmed3f(hp0, 0.f, sum), sum);
+    const float hmn = fmaxf(__builtin_amdgcn_fmed3f(hm0, 0.f, sum), 0.f);
+    float sx = half_slope<Textbook>(dp.qx, dm.qx);
+    float sy = half_slope<Textbook>(dp.qy, dm.qy);
+    s.hp = hp;
+    s.hm = hmn;
+    s.qxp = qx + sx;
+    s.qxm = qx - sx;
+    s.qyp = qy + sy;
+    s.qym = qy - sy;
+    return s;
+}
+
 // h*, qx*, qy*: cell and neighbours along the pass (m = minus, p = plus);
 // b*: cell-centre beds; Bp/Bm: the cell's east/west (north/south) face beds.
 template <bool Textbook>

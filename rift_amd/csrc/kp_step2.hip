@@ -802,21 +802,26 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
     R[1] = finish<Stage, Up, FM>(d, L, rD, j + step, dt, inv_dt, dcp, pp);
     // the row behind C (its south face bed is read as by_behind below)
     R[2].by = B.by;
+    // differences across the face between rows D and E (north minus south),
+    // carried to the next row, whose construction reads the same face
+    FaceDiff DY;
     {
         const Row& C = R[0];
         const Row& D = R[1];
         if constexpr (!Up) {
             const FaceStates yB = construct<Textbook>(A.h, B.h, C.h, A.b, B.b, C.b, A.hu, B.hu, C.hu,
                                                       A.hv, B.hv, C.hv, C.by, B.by);
-            Y[0] = construct<Textbook>(B.h, C.h, D.h, B.b, C.b, D.b, B.hu, C.hu, D.hu, B.hv, C.hv, D.hv,
-                                       D.by, C.by);
+            const FaceDiff dBC = face_diff(B.h, C.h, B.b, C.b, B.hu, C.hu, B.hv, C.hv);
+            DY = face_diff(C.h, D.h, C.b, D.b, C.hu, D.hu, C.hv, D.hv);
+            Y[0] = construct_d<Textbook>(C.h, C.b, C.hu, C.hv, DY, dBC, D.by, C.by);
             G[0] = face_flux<1>(yB.hp, yB.qxp, yB.qyp, Y[0].hm, Y[0].qxm, Y[0].qym, kappa);
         } else {
             // rows: A = j+2, B = j+1 (north of C), D = j-1
             const FaceStates yB = construct<Textbook>(C.h, B.h, A.h, C.b, B.b, A.b, C.hu, B.hu, A.hu,
                                                       C.hv, B.hv, A.hv, A.by, B.by);
-            Y[0] = construct<Textbook>(D.h, C.h, B.h, D.b, C.b, B.b, D.hu, C.hu, B.hu, D.hv, C.hv, B.hv,
-                                       B.by, C.by);
+            const FaceDiff dCB = face_diff(C.h, B.h, C.b, B.b, C.hu, B.hu, C.hv, B.hv);
+            DY = face_diff(D.h, C.h, D.b, C.b, D.hu, C.hu, D.hv, C.hv);
+            Y[0] = construct_d<Textbook>(C.h, C.b, C.hu, C.hv, dCB, DY, B.by, C.by);
             G[0] = face_flux<1>(Y[0].hp, Y[0].qxp, Y[0].qyp, yB.hm, yB.qxm, yB.qym, kappa);
         }
     }
@@ -850,12 +855,14 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         const Row& D = R[kd];
         const Row& E = R[ke];
         if constexpr (!Up) {
-            Y[kd] = construct<Textbook>(C.h, D.h, E.h, C.b, D.b, E.b, C.hu, D.hu, E.hu, C.hv, D.hv, E.hv,
-                                        E.by, D.by);
+            const FaceDiff dDE = face_diff(D.h, E.h, D.b, E.b, D.hu, E.hu, D.hv, E.hv);
+            Y[kd] = construct_d<Textbook>(D.h, D.b, D.hu, D.hv, dDE, DY, E.by, D.by);
+            DY = dDE;
             G[kd] = face_flux<1>(Y[kc].hp, Y[kc].qxp, Y[kc].qyp, Y[kd].hm, Y[kd].qxm, Y[kd].qym, kappa);
         } else {
-            Y[kd] = construct<Textbook>(E.h, D.h, C.h, E.b, D.b, C.b, E.hu, D.hu, C.hu, E.hv, D.hv, C.hv,
-                                        C.by, D.by);
+            const FaceDiff dED = face_diff(E.h, D.h, E.b, D.b, E.hu, D.hu, E.hv, D.hv);
+            Y[kd] = construct_d<Textbook>(D.h, D.b, D.hu, D.hv, DY, dED, C.by, D.by);
+            DY = dED;
             G[kd] = face_flux<1>(Y[kd].hp, Y[kd].qxp, Y[kd].qyp, Y[kc].hm, Y[kc].qxm, Y[kc].qym, kappa);
         }
         // north / south faces of row j
@@ -866,10 +873,11 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         // x pass of row j
         const float bxw = C.bx;
         const float bxe = from_right(bxw);
-        const FaceStates xs = construct<Textbook>(
-            from_left(C.h), C.h, from_right(C.h), from_left(C.b), C.b, from_right(C.b),
-            from_left(C.hu), C.hu, from_right(C.hu), from_left(C.hv), C.hv, from_right(C.hv),
-            bxe, bxw);
+        // the west face's w difference is the west neighbour's east one
+        const FaceDiff dE = face_diff(C.h, from_right(C.h), C.b, from_right(C.b), C.hu, from_right(C.hu), C.hv,
+                                      from_right(C.hv));
+        const FaceDiff dW{from_left(dE.w), C.hu - from_left(C.hu), C.hv - from_left(C.hv)};
+        const FaceStates xs = construct_d<Textbook>(C.h, C.b, C.hu, C.hv, dE, dW, bxe, bxw);
         const Flux fE = face_flux<0>(xs.hp, xs.qxp, xs.qyp, from_right(xs.hm), from_right(xs.qxm),
                                      from_right(xs.qym), kappa);
         const float fw0 = from_left(fE.f0), fw1 = from_left(fE.f1), fw2 = from_left(fE.f2);
