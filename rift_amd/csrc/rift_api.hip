@@ -975,6 +975,7 @@ int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
         // warm-up from the snapshot (captures the variant's graphs), then
         // the timed run from the same snapshot
         c->per_phase = g->per_phase;
+        c->use_graphs = g->use_graphs;   // timed the way the handle steps
         if ((rc = copy_state(g, c)) || (rc = rift_set_option(c, RIFT_OPT_VARIANT, v))) break;
         if ((rc = rift_step(c, nsteps)) || (rc = copy_state(g, c))) break;
         if ((rc = rift_step(c, 2))) break;
