@@ -190,9 +190,9 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
  * ensemble with any member at 0 launches its steps one by one too). */
 #define RIFT_OPT_GRAPHS 2
 /* RIFT_OPT_VARIANT: launch variant of the fused step's marching kernels,
- * 0 .. RIFT_NVARIANTS-1 (default 0): bit 0 = stage 1 with a larger register
- * budget (4 instead of 5 waves per SIMD), bit 1 = stage 2 likewise (3
- * instead of 4), bit 2 = work shares dealt to workgroups in launch order
+ * 0 .. RIFT_NVARIANTS-1 (default 0): bit 0 = stage 1 at 5 waves per SIMD
+ * instead of 4 (a smaller register budget), bit 1 = stage 2 at 4 instead of
+ * 3, bit 2 = work shares dealt to workgroups in launch order
  * instead of by XCD, bits 3-4 = least rows a wave marches in one share (the
  * cells per lane of a share: 14, 7, 28 or 42), bit 5 = every active unit
  * marched by a wave of its own (no workgroups of four neighbouring strips).
