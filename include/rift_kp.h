@@ -203,6 +203,11 @@ int rift_apply_boundaries(rift_grid* g); /* ApplyBoundaries, grid.cu:1408   */
  * RIFT_MASS_BUDGET at creation), 0 = without; switching it on resets the
  * budget (rift_budget_reset).  The state arithmetic is the same either way. */
 #define RIFT_OPT_MASS_BUDGET 4
+/* RIFT_OPT_DT_KERNEL: 0 (default) = the last workgroup of the fused step's
+ * stage-1 march forms dt (ComputeTimestep, simulator.cu:540-566) and the
+ * next step's forcing; 1 = a dt kernel of its own between the two marches.
+ * Both give the same dt, state and forcing bit for bit. */
+#define RIFT_OPT_DT_KERNEL 5
 int rift_set_option(rift_grid* g, int option, int value);
 
 /* Autotuner (no counterpart in the reference, whose 16 x 14 launch shape is

@@ -87,6 +87,9 @@ struct DtPartial {
     unsigned long long cells;
 };
 constexpr int kDtMaxBlocks = 512;   // workgroups of the dt kernel (partials per handle)
+// Slots of the dt reduction formed in the stage-1 march (128 bytes apart),
+// which grow and the march add into.
+constexpr int kDtSlots = 256;
 
 // Surge window of one step: map indices of the two planes and the factor.
 struct SurgeWindow {
@@ -126,6 +129,10 @@ struct Dev {
     float *mx1, *my1;            // fused step: stage-1 maxima of this step
     Scalars* sc;
     DtPartial* dtp;              // kDtMaxBlocks partials of the fused dt kernel
+    // single domain, dt formed by the stage-1 march: kDtSlots slots of 32
+    // words (vx, vy bits, active tiles, -, cells as 64 bits)
+    unsigned* gs;
+    int dt_in_march;             // launch argument: the stage-1 march forms dt
     SeriesTable* ser;            // [3]: hyetograph, drain, hydrograph
     int P, nx, ny, nbx, nby, ntiles;
     float inv_dx, inv_dy, kappa, psi_dtheta, nodata;
@@ -325,9 +332,14 @@ void launch_beds_from_corners(const Dev& d, int R, hipStream_t s);
 // Fused step (kp_step2.hip).  One wavefront marches one unit: 60 output
 // columns (lanes 2..61) by one 14-row tile row.
 constexpr int kStrip = 60;
-void launch_grow2(const Dev& d, hipStream_t s);
+// dt_in_march: grow adds its part of the dt reduction into Dev::gs for a
+// stage-1 march launched with dt_in_march
+void launch_grow2(const Dev& d, hipStream_t s, bool dt_in_march = false);
 void launch_timestep2(const Dev& d, hipStream_t s);
-void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks);
+// With dt_in_march (stage 1, single domain, after launch_grow2 with
+// dt_in_march), the last workgroup of the stage-1 march forms dt and no dt
+// kernel runs.
+void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks, bool dt_in_march = false);
 // workgroups of a marching kernel resident at once on a device of `cus` CUs
 // (for the instantiation d's flags and surge series select)
 int march_resident_blocks(const Dev& d, int stage, int variant, int cus);

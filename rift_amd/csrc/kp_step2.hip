@@ -98,8 +98,9 @@ __device__ __forceinline__ bool finished(const Dev& d)
     return d.tf > 0.0 && d.sc->t >= d.tf;
 }
 
+// Returns whether tile t is active in this step (false past the tiles).
 template <bool Mark>
-__device__ __forceinline__ void grow_tiles_body(const Dev& d, int t)
+__device__ __forceinline__ bool grow_tiles_body(const Dev& d, int t)
 {
     if (t == 0) {
         Scalars* sc = d.sc;
@@ -120,14 +121,14 @@ __device__ __forceinline__ void grow_tiles_body(const Dev& d, int t)
             if (d.sg_series) check_surge_slots(d, sc);
         }
     }
-    if (t >= d.ntiles) return;
+    if (t >= d.ntiles) return false;
     if (d.flip[t]) {   // stage 2 of the last step wrote the other buffer
         d.sel[t] ^= 1;
         d.flip[t] = 0;
     }
     const bool a = dilated(d, t);
     d.act[t] = a ? 1 : 0;
-    if (!a) return;
+    if (!a) return false;
     d.mx[t] = 0.f;
     d.my[t] = 0.f;
     d.mx1[t] = 0.f;
@@ -136,7 +137,7 @@ __device__ __forceinline__ void grow_tiles_body(const Dev& d, int t)
     // columns [16bx+2, 16bx+17]: s from 16bx/60 to (16bx+15)/60, each kept
     // only when the tile lies in that strip's tile range (strip_tiles)
     const int bx = t % d.nbx, by = t / d.nbx;
-    if (by >= d.nurows) return;
+    if (by >= d.nurows) return true;
     const int s0 = (kTileW * bx) / kStrip, s1 = min((kTileW * bx + kTileW - 1) / kStrip, d.nstrips - 1);
     const int tag = Mark ? 0 : d.sc->ustep + 1;
     for (int s = s0; s <= s1; ++s) {
@@ -147,6 +148,7 @@ __device__ __forceinline__ void grow_tiles_body(const Dev& d, int t)
             else d.utag[s * d.nurows + by] = tag;
         }
     }
+    return true;
 }
 
 // Surge relevance of every unit for the window in force: can it override a
@@ -165,16 +167,16 @@ __device__ __forceinline__ void unit_surge_flags(const Dev& d, int u, int s, int
     d.sgu[u] = rel ? 1 : 0;
 }
 
-// List appends of a workgroup of 256: for each of N lists, the flagged
-// threads get consecutive slots (waves in order, lanes in order within a
-// wave) from one atomic on the list's counter per workgroup, not one per
-// wave (same-address atomics from ~1300 waves serialise); slot[k] is this
-// thread's slot in list k (valid where flag[k]).  Every thread of the
-// workgroup must call it.
+// List appends of a workgroup of up to 1024 threads: for each of N lists,
+// the flagged threads get consecutive slots (waves in order, lanes in order
+// within a wave) from one atomic on the list's counter per workgroup, not
+// one per wave (same-address atomics serialise: ~1300 of them, one per
+// wave, cost several microseconds); slot[k] is this thread's slot in list k
+// (valid where flag[k]).  Every thread of the workgroup must call it.
 template <int N>
 __device__ __forceinline__ void wg_append(const bool (&flag)[N], int* const (&counter)[N], int (&slot)[N])
 {
-    __shared__ int s_cnt[4][N];
+    __shared__ int s_cnt[16][N];
     __shared__ int s_base[N];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long lt = (1ull << lane) - 1ull;
@@ -186,8 +188,9 @@ __device__ __forceinline__ void wg_append(const bool (&flag)[N], int* const (&co
     }
     __syncthreads();
     if (threadIdx.x < N) {
-        const int k = threadIdx.x;
-        const int tot = s_cnt[0][k] + s_cnt[1][k] + s_cnt[2][k] + s_cnt[3][k];
+        const int k = threadIdx.x, nw = (int)(blockDim.x >> 6);
+        int tot = 0;
+        for (int w = 0; w < nw; ++w) tot += s_cnt[w][k];
         s_base[k] = tot ? atomicAdd(counter[k], tot) : 0;
     }
     __syncthreads();
@@ -252,16 +255,6 @@ __device__ __forceinline__ void compact_quads_body(const Dev& d, int u)
     wg_append<3>(f, ctr, slot);
     if (part) d.ulist[slot[1]] = u;
     if (head) d.qlist[slot[2]] = (s >> 2) * d.nurows + r;
-}
-
-__global__ __launch_bounds__(256) void k_grow_tiles(Dev d)
-{
-    grow_tiles_body<false>(d, blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-__global__ __launch_bounds__(256) void k_compact_quads(Dev d)
-{
-    compact_quads_body(d, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -575,9 +568,10 @@ __device__ __forceinline__ void shift_lane(const Dev& d, Lane& L, int s, int rr)
 // whose origin cell is interior (grid.cu:1160-1170): stage 1 resets the flag
 // of an active tile (to 1 without tile acceleration), stage 2 sets it where
 // a cell is wet.  Several waves may share a tile; every write is idempotent.
+// wv: the largest tile maxima the wave flushed so far (uniform).
 template <int Stage>
 __device__ __forceinline__ void flush_tiles(const Dev& d, int s, int rr, const Lane& L, bool out_lane,
-                                            float mxl, float myl, bool wetl, int lane)
+                                            float mxl, float myl, bool wetl, int lane, float2& wv)
 {
     float* const tmx = (Stage == 1) ? d.mx1 : d.mx;
     float* const tmy = (Stage == 1) ? d.my1 : d.my;
@@ -591,6 +585,8 @@ __device__ __forceinline__ void flush_tiles(const Dev& d, int s, int rr, const L
         if (__ballot(seg && L.act1) == 0ull) continue;   // uniform
         const float m1 = wave_max(seg ? mxl : 0.f);
         const float m2 = wave_max(seg ? myl : 0.f);
+        wv.x = fmaxf(wv.x, m1);
+        wv.y = fmaxf(wv.y, m2);
         const bool anyw = __ballot(seg && wetl) != 0ull;
         if (lane == 0) {
             atomicMax(reinterpret_cast<unsigned*>(tmx) + T, __float_as_uint(m1));
@@ -763,7 +759,7 @@ constexpr bool phased_rows() { return FM == kLeanMask || FM == kCoastMask; }
 // is instantiated once per phase, so that advancing the window renames
 // registers at compile time instead of copying ~40 values per row.
 template <int Stage, bool Textbook, bool Up, bool Budget, unsigned FM>
-__device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane)
+__device__ float2 march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane)
 {
     constexpr int step = Up ? -1 : 1;
     const int c = kStrip * s + lane;
@@ -827,6 +823,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
     }
 
     float mxl = 0.f, myl = 0.f;
+    float2 wv = make_float2(0.f, 0.f);   // maxima of the tiles flushed (uniform)
     bool wetl = false;
     BudgetAcc acc{0.f, 0.f, 0.f, 0.f};
     // output row j in phase k; returns whether the run is done
@@ -835,7 +832,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
         constexpr int kd = (kc + 1) % 3, ke = (kc + 2) % 3;
         const int rj = (j - 2) / kTileH;   // uniform
         if (rj != rr) {
-            flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane);
+            flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane, wv);
             mxl = 0.f;
             myl = 0.f;
             wetl = false;
@@ -918,7 +915,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             }
         }
     }
-    flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane);
+    flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane, wv);
     if constexpr (Budget) {
         Scalars* sc = d.sc;
         if constexpr (Stage == 1) {
@@ -930,6 +927,7 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
             if (S.fl & FM & (RIFT_DRAIN_AVG | RIFT_DRAIN_GRID)) wave_add_f64(&sc->b_drain, acc.drain, lane);
         }
     }
+    return wv;
 }
 
 // Work partition.  The launch holds exactly the waves that fit on the
@@ -959,10 +957,19 @@ __device__ void march_run(const Dev& d, const StepScalars& S, int s, int ja, int
 // 240 bytes of its 60 output columns.  Partial quads stay with single waves:
 // a workgroup per quad idled 19 % of its waves on the coastal-surge
 // workload at 3 h and ran 9 % slower despite 21 % less traffic.
-template <int Stage, bool Textbook, int Waves, bool Budget, int Kind>
-__global__ __launch_bounds__(256, Waves) void k_march(Dev d)
+// dt formed by the stage-1 march (defined with the dt kernel below).
+__device__ __forceinline__ void march1_dt(const Dev& d, float2 wv);
+
+// Returns the largest tile maxima this wave flushed (uniform).
+template <int Stage, bool Textbook, bool Budget, int Kind>
+__device__ __forceinline__ float2 march_body(const Dev& d)
 {
     constexpr unsigned FM = kind_mask(Kind);
+    float2 wv = make_float2(0.f, 0.f);
+    auto keep = [&](float2 v) {
+        wv.x = fmaxf(wv.x, v.x);
+        wv.y = fmaxf(wv.y, v.y);
+    };
     const int lane = threadIdx.x & 63;
     // the wavefront index is uniform across the wave: in a scalar register,
     // every unit and row index below is scalar too (SALU address math,
@@ -970,7 +977,7 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // stage 1 runs before the dt kernel moves the unit count
     const int nunits = min((Stage == 1) ? d.sc->n_units_acc : d.sc->n_units, d.nunits);
-    if (nunits <= 0) return;
+    if (nunits <= 0) return wv;
 
     const int G = (int)gridDim.x;
     int pos = (int)blockIdx.x;
@@ -1001,13 +1008,13 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
     const int m = min(W / d.nstrips, mlim);
     if (nunits == d.nunits && m >= 2) {
         const int mi = wp / d.nstrips, s = wp - mi * d.nstrips;
-        if (mi >= m) return;
+        if (mi >= m) return wv;
         const int ja = 2 + (int)((long long)rows * mi / m);
         const int jb = 2 + (int)((long long)rows * (mi + 1) / m);
-        if (ja >= jb) return;
-        if (mi & 1) march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane);
-        else march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
-        return;
+        if (ja >= jb) return wv;
+        if (mi & 1) keep(march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane));
+        else keep(march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane));
+        return wv;
     }
 
     // full quads to the first GA workgroups, the other active units to the
@@ -1028,9 +1035,9 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
     // shares [i0, i1) of this workgroup / wave (at most one when nshare <= nwork)
     const long long i0 = ((long long)me * nshare + nwork - 1) / nwork;
     const long long i1 = ((long long)(me + 1) * nshare + nwork - 1) / nwork;
-    if (i0 >= i1) return;
+    if (i0 >= i1) return wv;
     const long long p0 = total * i0 / nshare, pend = total * i1 / nshare;
-    if (p0 >= pend) return;
+    if (p0 >= pend) return wv;
     int k = (int)(p0 / kTileH);
     int a = (int)(p0 - (long long)k * kTileH);
     for (;;) {
@@ -1051,10 +1058,20 @@ __global__ __launch_bounds__(256, Waves) void k_march(Dev d)
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (long long)kk * kTileH), d.ny - 2);
         // a quad's wave w marches strip 4c + w
         const int s = quad ? 4 * c + wid : c;
-        if (ja < jb) march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
+        if (ja < jb) keep(march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane));
         k = kk + 1;
         a = 0;
         if ((long long)k * kTileH >= pend) break;
+    }
+    return wv;
+}
+
+template <int Stage, bool Textbook, int Waves, bool Budget, int Kind>
+__global__ __launch_bounds__(256, Waves) void k_march(Dev d)
+{
+    const float2 wv = march_body<Stage, Textbook, Budget, Kind>(d);
+    if constexpr (Stage == 1) {
+        if (d.dt_in_march) march1_dt(d, wv);
     }
 }
 
@@ -1142,83 +1159,55 @@ __device__ __forceinline__ DtRed dt_block_reduce(DtRed r, DtRed* lds4)
     return r;
 }
 
-__device__ __forceinline__ void timestep2_body(const Dev& d)
+// Per-XCD ticket x of the dt kernel: a counter on its own 128-byte line
+// after the partial records.
+__device__ __forceinline__ unsigned* dt_xticket(const Dev& d, int x)
 {
-    __shared__ DtRed red4[4];
-    __shared__ DtRed total;
-    __shared__ bool last, ended;
-    __shared__ float s_tf;
-    __shared__ int s_match[3];
-    DtRed r{0.f, 0.f, 0ull, 0};
-    const int tid = threadIdx.x;
-    Scalars* sc = d.sc;
-    // the tail runs on a copy of the device scalars in LDS, staged by every
-    // workgroup up front (in flight with its tile loads; only the last one
-    // uses it) and written back by the last one: no round trip in the tail.
-    // Nothing changes the scalars while this kernel runs but the ticket,
-    // which the tail clears.
-    __shared__ Scalars ssc;
+    return reinterpret_cast<unsigned*>(d.dtp + kDtMaxBlocks) + 32 * x;
+}
+
+// Two-level ticket of a grid-wide reduction (thread 0 of each workgroup,
+// after its partial is stored and fenced): the workgroups of each XCD
+// (blockIdx % 8) count on a line of their own, the last of each on the
+// scalars' ticket (same-address atomics serialise: ~300 on one line took
+// ~6 us).  True in the last workgroup of the grid; every counter is back at
+// zero once the tail has run (the tail clears the scalars' ticket).
+__device__ __forceinline__ bool grid_ticket(const Dev& d)
+{
+    const int G = (int)gridDim.x, ng = min(G, 8), x = (int)blockIdx.x % ng;
+    const int nx = G / ng + (x < G % ng ? 1 : 0);
+    unsigned* xt = dt_xticket(d, x);
+    if (atomicAdd(xt, 1u) != (unsigned)(nx - 1)) return false;
+    __hip_atomic_store(xt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    return atomicAdd(&d.sc->ticket, 1u) == (unsigned)(ng - 1);
+}
+
+// Copy of the device scalars into LDS (every thread of the workgroup takes
+// part; visible after the next barrier).
+__device__ __forceinline__ void stage_scalars(const Dev& d, Scalars& ssc)
+{
     constexpr int kWords = (int)(sizeof(Scalars) / sizeof(unsigned));
     static_assert(sizeof(Scalars) % sizeof(unsigned) == 0, "Scalars is copied in 32-bit words");
-    for (int k = tid; k < kWords; k += 256)
-        reinterpret_cast<unsigned*>(&ssc)[k] = reinterpret_cast<const unsigned*>(sc)[k];
-    static_assert(kDtTilesPerBlock == 4 * 256, "four consecutive tiles per lane");
-    // four consecutive tiles per lane: one 32-bit load of the activity bytes
-    // and one 128-bit load of each maxima array, all in flight together (the
-    // per-tile arrays carry 64 bytes of padding, so the last lane's vector
-    // reads stay inside the allocation)
-    for (int base = blockIdx.x * kDtTilesPerBlock; base < d.ntiles; base += gridDim.x * kDtTilesPerBlock) {
-        const int t0 = base + 4 * tid;
-        if (t0 < d.ntiles) {
-            const uchar4 a4 = *reinterpret_cast<const uchar4*>(d.act + t0);
-            const float4 x1 = *reinterpret_cast<const float4*>(d.mx1 + t0);
-            const float4 x0 = *reinterpret_cast<const float4*>(d.mx + t0);
-            const float4 y1 = *reinterpret_cast<const float4*>(d.my1 + t0);
-            const float4 y0 = *reinterpret_cast<const float4*>(d.my + t0);
-            const bool a[4] = {a4.x != 0, a4.y != 0, a4.z != 0, a4.w != 0};
-            const float ax1[4] = {x1.x, x1.y, x1.z, x1.w}, ax0[4] = {x0.x, x0.y, x0.z, x0.w};
-            const float ay1[4] = {y1.x, y1.y, y1.z, y1.w}, ay0[4] = {y0.x, y0.y, y0.z, y0.w};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int t = t0 + k;
-                if (t >= d.ntiles) break;
-                r.vx = fmaxf(r.vx, a[k] ? ax1[k] : ax0[k]);
-                r.vy = fmaxf(r.vy, a[k] ? ay1[k] : ay0[k]);
-                if (a[k]) {
-                    const int bx = t % d.nbx, by = t / d.nbx;
-                    const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
-                    const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
-                    if (i1 > i0 && j1 > j0) r.cells += (unsigned long long)((i1 - i0) * (j1 - j0));
-                    ++r.nact;
-                }
-            }
-        }
-    }
-    r = dt_block_reduce(r, red4);
-    if (tid == 0) {
-        DtPartial* p = d.dtp + blockIdx.x;
-        __hip_atomic_store(&p->vx, r.vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->vy, r.vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->cells, r.cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->nact, r.nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        last = atomicAdd(&sc->ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    r = DtRed{0.f, 0.f, 0ull, 0};
-    for (int b = tid; b < (int)gridDim.x; b += 256) {
-        const DtPartial* p = d.dtp + b;
-        r.vx = fmaxf(r.vx, __hip_atomic_load(&p->vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        r.vy = fmaxf(r.vy, __hip_atomic_load(&p->vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        r.cells += __hip_atomic_load(&p->cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        r.nact += __hip_atomic_load(&p->nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();   // red4 is reused
-    r = dt_block_reduce(r, red4);
-    if (tid == 0) total = r;
-    __syncthreads();
+    for (int k = threadIdx.x; k < kWords; k += blockDim.x)
+        reinterpret_cast<unsigned*>(&ssc)[k] = reinterpret_cast<const unsigned*>(d.sc)[k];
+}
+
+// The tail of the dt reduction, on the one workgroup that completes it:
+// `tot` (thread 0) holds the maxima over every tile and the counts of the
+// active tiles and cells; ssc is the scalars' copy in LDS, written back at
+// the end.  ComputeTimestep's arithmetic, the unit counts moved to stage 2,
+// and the next step's forcing at the new time (UpdateSource runs at the
+// start of the next RunSimulation, simulator.cu:780-784, with the t of this
+// step).
+__device__ __forceinline__ void dt_tail(const Dev& d, const DtRed& tot, Scalars& ssc)
+{
+    __shared__ bool ended;
+    __shared__ float s_tf;
+    __shared__ int s_match[3];
+    const int tid = threadIdx.x;
+    Scalars* sc = d.sc;
+    constexpr int kWords = (int)(sizeof(Scalars) / sizeof(unsigned));
     if (tid == 0) {
         ssc.ticket = 0u;
         ended = ssc.finished != 0;
@@ -1231,17 +1220,17 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
             ssc.next_valid = 0;
             ssc.b_flux1 = 0.0;
         } else {
-            ssc.n_active = total.nact;
-            ssc.wet_cells = total.cells;
-            ssc.wcu_total += total.cells;
-            finish_timestep(d, total.vx, total.vy, &ssc);
+            ssc.n_active = tot.nact;
+            ssc.wet_cells = tot.cells;
+            ssc.wcu_total += tot.cells;
+            finish_timestep(d, tot.vx, tot.vy, &ssc);
             s_tf = (float)ssc.t;
         }
         s_match[0] = s_match[1] = s_match[2] = -1;
     }
     __syncthreads();
     if (ended) {
-        for (int k = tid; k < kWords; k += 256)
+        for (int k = tid; k < kWords; k += blockDim.x)
             reinterpret_cast<unsigned*>(sc)[k] = reinterpret_cast<const unsigned*>(&ssc)[k];
         return;
     }
@@ -1254,7 +1243,7 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
         if (!use[k]) continue;
         const SeriesTable& st = d.ser[k];
         const int n = st.n;
-        for (int i = tid; i < n - 1; i += 256)
+        for (int i = tid; i < n - 1; i += blockDim.x)
             if (tf >= st.time[i] && tf < st.time[i + 1]) atomicMax(&s_match[k], i);
     }
     __syncthreads();
@@ -1279,8 +1268,195 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
         ssc.next_valid = 1;
     }
     __syncthreads();
-    for (int k = tid; k < kWords; k += 256)
+    for (int k = tid; k < kWords; k += blockDim.x)
         reinterpret_cast<unsigned*>(sc)[k] = reinterpret_cast<const unsigned*>(&ssc)[k];
+}
+
+// Interior cells of tile t (the cells a step updates when it is active).
+__device__ __forceinline__ unsigned long long tile_cells(const Dev& d, int t)
+{
+    const int bx = t % d.nbx, by = t / d.nbx;
+    const int i0 = bx * kTileW + 2, j0 = by * kTileH + 2;
+    const int i1 = min(i0 + kTileW, d.nx - 2), j1 = min(j0 + kTileH, d.ny - 2);
+    return (i1 > i0 && j1 > j0) ? (unsigned long long)((i1 - i0) * (j1 - j0)) : 0ull;
+}
+
+__device__ __forceinline__ void timestep2_body(const Dev& d)
+{
+    __shared__ DtRed red4[4];
+    __shared__ bool last;
+    DtRed r{0.f, 0.f, 0ull, 0};
+    const int tid = threadIdx.x;
+    // the tail runs on a copy of the device scalars in LDS, staged by every
+    // workgroup up front (in flight with its tile loads; only the last one
+    // uses it) and written back by the last one: no round trip in the tail.
+    // Nothing changes the scalars while this kernel runs but the ticket,
+    // which the tail clears.
+    __shared__ Scalars ssc;
+    stage_scalars(d, ssc);
+    static_assert(kDtTilesPerBlock == 4 * 256, "four consecutive tiles per lane");
+    // four consecutive tiles per lane: one 32-bit load of the activity bytes
+    // and one 128-bit load of each maxima array, all in flight together (the
+    // per-tile arrays carry 64 bytes of padding, so the last lane's vector
+    // reads stay inside the allocation)
+    for (int base = blockIdx.x * kDtTilesPerBlock; base < d.ntiles; base += gridDim.x * kDtTilesPerBlock) {
+        const int t0 = base + 4 * tid;
+        if (t0 < d.ntiles) {
+            const uchar4 a4 = *reinterpret_cast<const uchar4*>(d.act + t0);
+            const float4 x1 = *reinterpret_cast<const float4*>(d.mx1 + t0);
+            const float4 x0 = *reinterpret_cast<const float4*>(d.mx + t0);
+            const float4 y1 = *reinterpret_cast<const float4*>(d.my1 + t0);
+            const float4 y0 = *reinterpret_cast<const float4*>(d.my + t0);
+            const bool a[4] = {a4.x != 0, a4.y != 0, a4.z != 0, a4.w != 0};
+            const float ax1[4] = {x1.x, x1.y, x1.z, x1.w}, ax0[4] = {x0.x, x0.y, x0.z, x0.w};
+            const float ay1[4] = {y1.x, y1.y, y1.z, y1.w}, ay0[4] = {y0.x, y0.y, y0.z, y0.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int t = t0 + k;
+                if (t >= d.ntiles) break;
+                r.vx = fmaxf(r.vx, a[k] ? ax1[k] : ax0[k]);
+                r.vy = fmaxf(r.vy, a[k] ? ay1[k] : ay0[k]);
+                if (a[k]) {
+                    r.cells += tile_cells(d, t);
+                    ++r.nact;
+                }
+            }
+        }
+    }
+    r = dt_block_reduce(r, red4);
+    if (tid == 0) {
+        DtPartial* p = d.dtp + blockIdx.x;
+        __hip_atomic_store(&p->vx, r.vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->vy, r.vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->cells, r.cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&p->nact, r.nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        last = grid_ticket(d);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    r = DtRed{0.f, 0.f, 0ull, 0};
+    for (int b = tid; b < (int)gridDim.x; b += 256) {
+        const DtPartial* p = d.dtp + b;
+        r.vx = fmaxf(r.vx, __hip_atomic_load(&p->vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        r.vy = fmaxf(r.vy, __hip_atomic_load(&p->vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        r.cells += __hip_atomic_load(&p->cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        r.nact += __hip_atomic_load(&p->nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();   // red4 is reused
+    r = dt_block_reduce(r, red4);
+    dt_tail(d, r, ssc);
+}
+
+// Two-level ticket as grid_ticket, for callers whose data travels in
+// read-modify-write atomics only (which every workgroup sees without a
+// fence): no fence, so no write-back of the L2 by every workgroup.
+__device__ __forceinline__ bool grid_ticket_rmw(const Dev& d)
+{
+    const int G = (int)gridDim.x, ng = min(G, 8), x = (int)blockIdx.x % ng;
+    const int nx = G / ng + (x < G % ng ? 1 : 0);
+    unsigned* xt = dt_xticket(d, x);
+    if (atomicAdd(xt, 1u) != (unsigned)(nx - 1)) return false;
+    atomicExch(xt, 0u);
+    return atomicAdd(&d.sc->ticket, 1u) == (unsigned)(ng - 1);
+}
+
+// dt formed by the stage-1 march itself (Dev::dt_in_march; single domain):
+// the march's waves keep the largest maxima they flush into the active
+// tiles; each workgroup takes the largest of its four waves into dt slot
+// blockIdx % kDtSlots (atomic maxima), where grow left the stale maxima of
+// the tiles it found inactive and the active tile and cell counts; the last
+// workgroup to finish (grid_ticket_rmw) reads and clears the slots, one per
+// thread, and runs the dt kernel's tail.  The same maxima and counts as the
+// dt kernel's pass over every tile: every active tile's stage-1 maxima are
+// the largest of the values the waves flushed into it (grow cleared them),
+// and an inactive tile contributes its stale maxima.  Saves the dt kernel's
+// launch and its pass over the tile arrays.  Every value crossing
+// workgroups travels in read-modify-write atomics, and a workgroup takes its
+// ticket only after its own have returned (its waves' budget atomics
+// included): a device-scope fence here would write the L2 back in every
+// workgroup.  The tail's loads (slots, scalars, the boundary outflow, the
+// forcing series' times) are issued together.
+__device__ __forceinline__ void march1_dt(const Dev& d, float2 wv)
+{
+    static_assert(kDtSlots == 256, "one dt slot per thread of the tail");
+    __shared__ float s_w[4][2];
+    __shared__ bool s_last;
+    __shared__ DtRed red4[4];
+    __shared__ Scalars ssc;
+    const int tid = threadIdx.x;
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's memory operations completed
+    if ((tid & 63) == 0) {
+        s_w[tid >> 6][0] = wv.x;
+        s_w[tid >> 6][1] = wv.y;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float vx = s_w[0][0], vy = s_w[0][1];
+        for (int w = 1; w < 4; ++w) {
+            vx = fmaxf(vx, s_w[w][0]);
+            vy = fmaxf(vy, s_w[w][1]);
+        }
+        unsigned* g = d.gs + 32 * (blockIdx.x % kDtSlots);
+        const unsigned o1 = vx > 0.f ? atomicMax(g, __float_as_uint(vx)) : 0u;
+        const unsigned o2 = vy > 0.f ? atomicMax(g + 1, __float_as_uint(vy)) : 0u;
+        // the ticket is taken once both maxima have returned
+        asm volatile("" ::"v"(o1), "v"(o2) : "memory");
+        s_last = grid_ticket_rmw(d);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    // slot tid, read and cleared for the next step's grow
+    unsigned* g = d.gs + 32 * tid;
+    const unsigned bx = atomicExch(g, 0u), by = atomicExch(g + 1, 0u), na = atomicExch(g + 2, 0u);
+    const unsigned long long nc = atomicExch(reinterpret_cast<unsigned long long*>(g + 4), 0ull);
+    // the scalars did not change in this launch but for the ticket and the
+    // stage-1 boundary outflow (atomics of every workgroup, read here)
+    double bf = 0.0;
+    if (tid == 0 && (d.flags & RIFT_MASS_BUDGET)) bf = atomicAdd(&d.sc->b_flux1, 0.0);
+    stage_scalars(d, ssc);
+    DtRed r{__uint_as_float(bx), __uint_as_float(by), nc, (int)na};
+    r = dt_block_reduce(r, red4);   // its barrier also publishes ssc
+    if (tid == 0 && (d.flags & RIFT_MASS_BUDGET)) ssc.b_flux1 = bf;
+    dt_tail(d, r, ssc);
+}
+
+// Single domain, one thread per tile: grow; when the stage-1 march forms dt
+// (Dev::dt_in_march), also the workgroup's share of the dt reduction over
+// its tiles (stale maxima of the inactive ones, counts of the active ones
+// and their interior cells), added by atomics into dt slot blockIdx %
+// kDtSlots (one 128-byte line each, so that few workgroups meet on a line),
+// which the march's dt tail reads and clears.
+__global__ __launch_bounds__(256) void k_grow_tiles(Dev d)
+{
+    __shared__ DtRed red4[4];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool a = grow_tiles_body<false>(d, t);
+    if (!d.dt_in_march) return;   // uniform
+    DtRed r{0.f, 0.f, 0ull, 0};
+    if (a) {
+        r.cells = tile_cells(d, t);
+        r.nact = 1;
+    } else if (t < d.ntiles) {
+        r.vx = d.mx[t];
+        r.vy = d.my[t];
+    }
+    r = dt_block_reduce(r, red4);
+    if (threadIdx.x == 0) {
+        unsigned* g = d.gs + 32 * (blockIdx.x % kDtSlots);
+        if (r.vx > 0.f) atomicMax(g, __float_as_uint(r.vx));
+        if (r.vy > 0.f) atomicMax(g + 1, __float_as_uint(r.vy));
+        if (r.nact) {
+            atomicAdd(g + 2, (unsigned)r.nact);
+            atomicAdd(reinterpret_cast<unsigned long long*>(g + 4), r.cells);
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_compact_quads(Dev d)
+{
+    compact_quads_body(d, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_timestep2(Dev d)
@@ -1293,10 +1469,12 @@ void launch_timestep2(const Dev& d, hipStream_t s)
     hipLaunchKernelGGL(k_timestep2, dim3(dt_blocks(d.ntiles)), dim3(256), 0, s, d);
 }
 
-void launch_grow2(const Dev& d, hipStream_t s)
+void launch_grow2(const Dev& d, hipStream_t s, bool dt_in_march)
 {
-    hipLaunchKernelGGL(k_grow_tiles, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_compact_quads, dim3((d.nunits + 255) / 256), dim3(256), 0, s, d);
+    Dev arg = d;
+    arg.dt_in_march = dt_in_march ? 1 : 0;
+    hipLaunchKernelGGL(k_grow_tiles, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, arg);
+    hipLaunchKernelGGL(k_compact_quads, dim3((d.nunits + 1023) / 1024), dim3(1024), 0, s, d);
 }
 
 // Launch variants (all give identical results; they differ in register
@@ -1345,10 +1523,11 @@ static const void* march_kernel(int stage, int variant, int kind, unsigned flags
     return (variant & 2) ? march_fn<2, 4>(kind, tb, bud) : march_fn<2, 3>(kind, tb, bud);
 }
 
-void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks)
+void launch_march(const Dev& d, int stage, hipStream_t s, int variant, int blocks, bool dt_in_march)
 {
     blocks = std::max(1, blocks);
     Dev arg = d;
+    arg.dt_in_march = (stage == 1 && dt_in_march) ? 1 : 0;
     void* args[] = {&arg};
     (void)hipLaunchKernel(march_kernel(stage, variant, march_kind(d), d.flags), dim3(blocks), dim3(256), args,
                           0, s);

@@ -84,6 +84,7 @@ struct rift_grid {
     // handle's Dev by value, so any change of it invalidates the graphs
     hipGraphExec_t graph1 = nullptr, graphN = nullptr;
     bool use_graphs = true;
+    bool dt_in_march = true;      // the stage-1 march forms dt (else the dt kernel)
     bool prof = false;
     int sg_dt_int = 0;            // device surge series: map interval, seconds
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_events;
@@ -394,10 +395,14 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.qlist = alloc_bytes<int>(g, (size_t)d.nquads * d.nurows);
     d.utag = alloc_bytes<int>(g, d.nunits);
     d.sc = alloc_bytes<Scalars>(g, 1);
-    d.dtp = alloc_bytes<DtPartial>(g, kDtMaxBlocks);
+    // the partial records, then 8 ticket counters a 128-byte line apart
+    d.dtp = alloc_bytes<DtPartial>(g, kDtMaxBlocks + (8 * 128) / sizeof(DtPartial) + 1);
+    // dt formed by the stage-1 march: the slots grow and the march add into
+    d.gs = alloc_bytes<unsigned>(g, 32 * (size_t)kDtSlots);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
-    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.dtp && d.ser && g->dsum && d.sel &&
+    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.dtp && d.gs && d.ser &&
+         g->dsum && d.sel &&
          d.flip && d.ulist && d.umark && d.qlist && d.utag;
     if (!ok) {
         rift_destroy(g);
@@ -784,16 +789,17 @@ static int step_fused(rift_grid* g)
 {
     Dev& d = g->d;
     prime(g);
+    const bool dt_in_march = g->dt_in_march;
     {
         ProfScope ps(g, 4);
         if (d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(d, g->stream);
-        launch_grow2(d, g->stream);
+        launch_grow2(d, g->stream, dt_in_march);
     }
     {
         ProfScope ps(g, 0);
-        launch_march(d, 1, g->stream, g->variant, g->march_blocks[0][g->variant]);
+        launch_march(d, 1, g->stream, g->variant, g->march_blocks[0][g->variant], dt_in_march);
     }
-    {
+    if (!dt_in_march) {
         ProfScope ps(g, 5);
         launch_timestep2(d, g->stream);
     }
@@ -887,6 +893,13 @@ int rift_set_option(rift_grid* g, int option, int value)
     case RIFT_OPT_GRAPHS:
         g->use_graphs = value != 0;
         return RIFT_OK;
+    case RIFT_OPT_DT_KERNEL:
+        if (value != 0 && value != 1) return fail(RIFT_E_ARG, "rift_set_option: dt kernel is 0 or 1");
+        if (g->dt_in_march == (value == 0)) return RIFT_OK;
+        HIPCHK(hipStreamSynchronize(g->stream));
+        drop_graphs(g);   // the graphs hold the launch sequence
+        g->dt_in_march = value == 0;
+        return prepare_graphs(g);
     case RIFT_OPT_MASS_BUDGET: {
         // the budget instantiation of the marching kernels on or off; the
         // accumulators restart from the current state when it comes on
@@ -976,6 +989,7 @@ int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
         // the timed run from the same snapshot
         c->per_phase = g->per_phase;
         c->use_graphs = g->use_graphs;   // timed the way the handle steps
+        c->dt_in_march = g->dt_in_march;
         if ((rc = copy_state(g, c)) || (rc = rift_set_option(c, RIFT_OPT_VARIANT, v))) break;
         if ((rc = rift_step(c, nsteps)) || (rc = copy_state(g, c))) break;
         if ((rc = rift_step(c, 2))) break;

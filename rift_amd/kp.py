@@ -36,7 +36,7 @@ FIELDS = ["w", "hu", "hv", "depth", "bc", "n", "K", "F", "dw", "dhu", "dhv", "G"
 SER_HYETO, SER_DRAIN, SER_HYDRO = 0, 1, 2
 MAP_RAIN, MAP_DRAIN, MAP_SURGE = 0, 1, 2
 MASS_BUDGET = 1 << 13
-OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT, OPT_MASS_BUDGET = 1, 2, 3, 4
+OPT_STEP_PATH, OPT_GRAPHS, OPT_VARIANT, OPT_MASS_BUDGET, OPT_DT_KERNEL = 1, 2, 3, 4, 5
 NVARIANTS = 64
 KERNEL_CLASSES = ["flux1", "stage2_flux", "integrate", "boundaries", "grow", "timestep"]
 
@@ -398,6 +398,12 @@ class Grid:
     def set_budget(self, on):
         """FP64 volume budget kernels on (resets the budget) or off."""
         _check(load().rift_set_option(self.h, OPT_MASS_BUDGET, 1 if on else 0), "rift_set_option")
+
+    def set_dt_kernel(self, on):
+        """True: the fused step forms dt in a kernel of its own between the
+        two marches; False (default): the stage-1 march's last workgroup
+        forms it.  Same results bit for bit."""
+        _check(load().rift_set_option(self.h, OPT_DT_KERNEL, 1 if on else 0), "rift_set_option")
 
     def autotune(self, nsteps=16):
         """Times every launch variant on a copy of the current state, selects

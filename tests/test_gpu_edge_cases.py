@@ -74,3 +74,35 @@ def test_oversized_domain_is_rejected(rift):
     refused before any allocation."""
     with pytest.raises(rift.RiftError, match="too large"):
         rift.Grid(40003, 40003, 1.0)
+
+
+def test_beds_that_are_no_corner_means_run_the_per_phase_path(rift):
+    """Face beds that are not two-point means of one corner field (no
+    reference geometry gives them) keep the uploaded planes: rift_step runs
+    the per-phase kernels, bit for bit as when that path is asked for, and
+    the autotuner (which times the fused step) refuses the grid."""
+    dem = _dem((40, 52), 4)
+    geo = rift.geometry(dem, n_const=0.03)
+    BX = geo["BX"].copy()
+    BX[20, 25] += 0.05   # one interior face off its corners
+    assert rift.corners_from_faces(BX, geo["BY"]) is None
+    h0 = np.maximum(18.0 - dem, 0.0)
+    geo_h = rift.geometry(dem, h0=h0, n_const=0.03)
+    grids = []
+    for per_phase in (False, True):
+        g = rift.Grid(geo["nx"], geo["ny"], 5.0, flags=rift.TILE_ACCEL | rift.H_INIT)
+        g.upload_static(BX, geo["BY"], geo["n"], None, geo_h["h0"])
+        g.init(0.0)
+        if per_phase:
+            g.set_path(True)
+        g.step(60)
+        grids.append(g)
+    a, b = grids
+    assert a.time == b.time
+    for f in ("depth", "hu", "hv"):
+        np.testing.assert_array_equal(a.field(f), b.field(f))
+    assert np.abs(a.field("hu")).max() > 0
+    with pytest.raises(rift.RiftError, match="corner"):
+        a.autotune(4)
+    for g in grids:
+        g.close()

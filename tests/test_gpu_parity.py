@@ -246,6 +246,54 @@ def test_graph_replays_match_eager_launches(rift):
     b.close()
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_dt_in_stage1_march_matches_dt_kernel(rift, graphs):
+    """dt formed by the last workgroup of the stage-1 march (default) equals
+    the dt kernel's (RIFT_OPT_DT_KERNEL 1) bit for bit: time and dt every
+    replay, state, tile sets, wet-cell counts and the next steps' forcing
+    (hyetograph, hydrograph), the FP64 budget terms to rounding, also with a smaller
+    active set (a dry start) and across a switch between the two forms."""
+    cfg = synth.rain_basin(200, seed=5)
+    N = cfg["dem"].shape[0] - 1
+    nx = N + 4
+    idx = [j * nx + i for i in range(60, 64) for j in range(40, 43)]
+    kw = dict(n_const=cfg["n_const"], K=cfg["K"], psi=cfg["psi"], dtheta=cfg["dtheta"],
+              flags=rift.TILE_ACCEL | rift.RAIN_AVG | rift.HYDROGRAPH | rift.MASS_BUDGET)
+    t, v = synth.series_hours_to_seconds(cfg["hyeto_hours"], cfg["hyeto_mmhr"])
+    th, vh = synth.series_hours_to_seconds(np.array([0.0, 0.02, 0.1], np.float32),
+                                           np.array([0.0, 80.0, 20.0], np.float32))
+    a = rift.grid_from_rasters(cfg["dem"], cfg["dx"], **kw)
+    b = rift.grid_from_rasters(cfg["dem"], cfg["dx"], **kw)
+    for g in (a, b):
+        g.set_series(rift.SER_HYETO, t, v)
+        g.set_source_cells(idx)
+        g.set_series(rift.SER_HYDRO, th, vh)
+        g.set_graphs(graphs)
+    b.set_dt_kernel(True)
+    for rep, n in enumerate((1, 16, 3, 16, 1, 19, 40)):
+        if rep == 5:
+            a.set_dt_kernel(True)    # switch forms mid-run: same trajectory
+            b.set_dt_kernel(False)
+        a.step(n)
+        b.step(n)
+        assert a.time == b.time, rep
+        assert a.wet_cells() == b.wet_cells(), rep
+        np.testing.assert_array_equal(a.rates(), b.rates())
+        for f in ("depth", "hu", "hv", "F"):
+            np.testing.assert_array_equal(a.field(f), b.field(f), err_msg=f"{f} after replay {rep}")
+        wa, aa, na = a.tiles()
+        wb, ab, nb = b.tiles()
+        assert na == nb, rep
+        np.testing.assert_array_equal(wa, wb)
+        np.testing.assert_array_equal(aa, ab)
+        # the budget terms are FP64 sums of the waves' atomics, in the order
+        # the waves arrive: equal to rounding in the last bits
+        np.testing.assert_allclose(a.budget(), b.budget(), rtol=1e-12, atol=1e-12)
+    assert a.time[0] > 0.0 and a.wet_cells()[1] > 0
+    a.close()
+    b.close()
+
+
 def test_launch_variants_bit_identical_and_autotuner(rift):
     """Every launch variant of the fused step (register budget per stage, work
     order) gives the default variant's state, dt and wet set bit for bit; the
