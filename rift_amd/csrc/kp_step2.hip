@@ -1166,21 +1166,45 @@ __device__ __forceinline__ unsigned* dt_xticket(const Dev& d, int x)
     return reinterpret_cast<unsigned*>(d.dtp + kDtMaxBlocks) + 32 * x;
 }
 
-// Two-level ticket of a grid-wide reduction (thread 0 of each workgroup,
-// after its partial is stored and fenced): the workgroups of each XCD
-// (blockIdx % 8) count on a line of their own, the last of each on the
-// scalars' ticket (same-address atomics serialise: ~300 on one line took
-// ~6 us).  True in the last workgroup of the grid; every counter is back at
-// zero once the tail has run (the tail clears the scalars' ticket).
-__device__ __forceinline__ bool grid_ticket(const Dev& d)
+// The dt reductions of the fused step (the dt kernel's, or the stage-1
+// march's) meet in kDtSlots slots of Dev::gs, one 128-byte line each: a
+// workgroup adds its maxima and counts into slot blockIdx % kDtSlots by
+// atomics and takes a two-level ticket (the workgroups of each XCD,
+// blockIdx % 8, count on a line of their own, the last of each on the
+// scalars' ticket: same-address atomics serialise, ~300 on one line took
+// ~6 us); the last workgroup of the grid reads and clears the slots, one per
+// thread.  Every value crossing workgroups travels in read-modify-write
+// atomics, and a workgroup takes its ticket only after its additions have
+// returned: no device-scope fence, which would write the XCD's L2 back in
+// every workgroup.  Thread 0 of each workgroup; true in the last one.  The
+// ticket counters are back at zero once the tail has run (the tail clears
+// the scalars' ticket).
+__device__ __forceinline__ bool dt_slot_add(const Dev& d, const DtRed& r)
 {
+    unsigned* g = d.gs + 32 * (blockIdx.x % kDtSlots);
+    const unsigned o1 = r.vx > 0.f ? atomicMax(g, __float_as_uint(r.vx)) : 0u;
+    const unsigned o2 = r.vy > 0.f ? atomicMax(g + 1, __float_as_uint(r.vy)) : 0u;
+    const unsigned o3 = r.nact ? atomicAdd(g + 2, (unsigned)r.nact) : 0u;
+    const unsigned long long o4 = r.nact ? atomicAdd(reinterpret_cast<unsigned long long*>(g + 4), r.cells) : 0ull;
+    // the ticket is taken once the additions have returned
+    asm volatile("" ::"v"(o1), "v"(o2), "v"(o3), "v"(o4) : "memory");
     const int G = (int)gridDim.x, ng = min(G, 8), x = (int)blockIdx.x % ng;
     const int nx = G / ng + (x < G % ng ? 1 : 0);
     unsigned* xt = dt_xticket(d, x);
     if (atomicAdd(xt, 1u) != (unsigned)(nx - 1)) return false;
-    __hip_atomic_store(xt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
+    atomicExch(xt, 0u);
     return atomicAdd(&d.sc->ticket, 1u) == (unsigned)(ng - 1);
+}
+
+// Slot `threadIdx.x` of the dt slots, read and cleared (the last workgroup,
+// every thread; kDtSlots == its threads).
+__device__ __forceinline__ DtRed dt_slot_take(const Dev& d)
+{
+    static_assert(kDtSlots == 256, "one dt slot per thread of the last workgroup");
+    unsigned* g = d.gs + 32 * threadIdx.x;
+    const unsigned bx = atomicExch(g, 0u), by = atomicExch(g + 1, 0u), na = atomicExch(g + 2, 0u);
+    const unsigned long long nc = atomicExch(reinterpret_cast<unsigned long long*>(g + 4), 0ull);
+    return DtRed{__uint_as_float(bx), __uint_as_float(by), nc, (int)na};
 }
 
 // Copy of the device scalars into LDS (every thread of the workgroup takes
@@ -1324,42 +1348,13 @@ __device__ __forceinline__ void timestep2_body(const Dev& d)
         }
     }
     r = dt_block_reduce(r, red4);
-    if (tid == 0) {
-        DtPartial* p = d.dtp + blockIdx.x;
-        __hip_atomic_store(&p->vx, r.vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->vy, r.vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->cells, r.cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&p->nact, r.nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        last = grid_ticket(d);
-    }
+    if (tid == 0) last = dt_slot_add(d, r);
     __syncthreads();
     if (!last) return;
-    __threadfence();
-    r = DtRed{0.f, 0.f, 0ull, 0};
-    for (int b = tid; b < (int)gridDim.x; b += 256) {
-        const DtPartial* p = d.dtp + b;
-        r.vx = fmaxf(r.vx, __hip_atomic_load(&p->vx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        r.vy = fmaxf(r.vy, __hip_atomic_load(&p->vy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        r.cells += __hip_atomic_load(&p->cells, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        r.nact += __hip_atomic_load(&p->nact, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    r = dt_slot_take(d);
     __syncthreads();   // red4 is reused
     r = dt_block_reduce(r, red4);
     dt_tail(d, r, ssc);
-}
-
-// Two-level ticket as grid_ticket, for callers whose data travels in
-// read-modify-write atomics only (which every workgroup sees without a
-// fence): no fence, so no write-back of the L2 by every workgroup.
-__device__ __forceinline__ bool grid_ticket_rmw(const Dev& d)
-{
-    const int G = (int)gridDim.x, ng = min(G, 8), x = (int)blockIdx.x % ng;
-    const int nx = G / ng + (x < G % ng ? 1 : 0);
-    unsigned* xt = dt_xticket(d, x);
-    if (atomicAdd(xt, 1u) != (unsigned)(nx - 1)) return false;
-    atomicExch(xt, 0u);
-    return atomicAdd(&d.sc->ticket, 1u) == (unsigned)(ng - 1);
 }
 
 // dt formed by the stage-1 march itself (Dev::dt_in_march; single domain):
@@ -1367,20 +1362,18 @@ __device__ __forceinline__ bool grid_ticket_rmw(const Dev& d)
 // tiles; each workgroup takes the largest of its four waves into dt slot
 // blockIdx % kDtSlots (atomic maxima), where grow left the stale maxima of
 // the tiles it found inactive and the active tile and cell counts; the last
-// workgroup to finish (grid_ticket_rmw) reads and clears the slots, one per
+// workgroup to finish (dt_slot_add) reads and clears the slots, one per
 // thread, and runs the dt kernel's tail.  The same maxima and counts as the
 // dt kernel's pass over every tile: every active tile's stage-1 maxima are
 // the largest of the values the waves flushed into it (grow cleared them),
 // and an inactive tile contributes its stale maxima.  Saves the dt kernel's
 // launch and its pass over the tile arrays.  Every value crossing
-// workgroups travels in read-modify-write atomics, and a workgroup takes its
-// ticket only after its own have returned (its waves' budget atomics
-// included): a device-scope fence here would write the L2 back in every
-// workgroup.  The tail's loads (slots, scalars, the boundary outflow, the
-// forcing series' times) are issued together.
+// workgroups travels in read-modify-write atomics (dt_slot_add), and every
+// wave waits for its own memory operations (the budget atomics among them)
+// before its workgroup takes the ticket.  The tail's loads (slots, scalars,
+// the boundary outflow) are issued together.
 __device__ __forceinline__ void march1_dt(const Dev& d, float2 wv)
 {
-    static_assert(kDtSlots == 256, "one dt slot per thread of the tail");
     __shared__ float s_w[4][2];
     __shared__ bool s_last;
     __shared__ DtRed red4[4];
@@ -1393,30 +1386,21 @@ __device__ __forceinline__ void march1_dt(const Dev& d, float2 wv)
     }
     __syncthreads();
     if (tid == 0) {
-        float vx = s_w[0][0], vy = s_w[0][1];
+        DtRed r{s_w[0][0], s_w[0][1], 0ull, 0};
         for (int w = 1; w < 4; ++w) {
-            vx = fmaxf(vx, s_w[w][0]);
-            vy = fmaxf(vy, s_w[w][1]);
+            r.vx = fmaxf(r.vx, s_w[w][0]);
+            r.vy = fmaxf(r.vy, s_w[w][1]);
         }
-        unsigned* g = d.gs + 32 * (blockIdx.x % kDtSlots);
-        const unsigned o1 = vx > 0.f ? atomicMax(g, __float_as_uint(vx)) : 0u;
-        const unsigned o2 = vy > 0.f ? atomicMax(g + 1, __float_as_uint(vy)) : 0u;
-        // the ticket is taken once both maxima have returned
-        asm volatile("" ::"v"(o1), "v"(o2) : "memory");
-        s_last = grid_ticket_rmw(d);
+        s_last = dt_slot_add(d, r);
     }
     __syncthreads();
     if (!s_last) return;
-    // slot tid, read and cleared for the next step's grow
-    unsigned* g = d.gs + 32 * tid;
-    const unsigned bx = atomicExch(g, 0u), by = atomicExch(g + 1, 0u), na = atomicExch(g + 2, 0u);
-    const unsigned long long nc = atomicExch(reinterpret_cast<unsigned long long*>(g + 4), 0ull);
+    DtRed r = dt_slot_take(d);
     // the scalars did not change in this launch but for the ticket and the
     // stage-1 boundary outflow (atomics of every workgroup, read here)
     double bf = 0.0;
     if (tid == 0 && (d.flags & RIFT_MASS_BUDGET)) bf = atomicAdd(&d.sc->b_flux1, 0.0);
     stage_scalars(d, ssc);
-    DtRed r{__uint_as_float(bx), __uint_as_float(by), nc, (int)na};
     r = dt_block_reduce(r, red4);   // its barrier also publishes ssc
     if (tid == 0 && (d.flags & RIFT_MASS_BUDGET)) ssc.b_flux1 = bf;
     dt_tail(d, r, ssc);
