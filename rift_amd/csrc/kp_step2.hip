@@ -1407,33 +1407,40 @@ __device__ __forceinline__ void march1_dt(const Dev& d, float2 wv)
 }
 
 // Single domain, one thread per tile: grow; when the stage-1 march forms dt
-// (Dev::dt_in_march), also the workgroup's share of the dt reduction over
-// its tiles (stale maxima of the inactive ones, counts of the active ones
-// and their interior cells), added by atomics into dt slot blockIdx %
-// kDtSlots (one 128-byte line each, so that few workgroups meet on a line),
-// which the march's dt tail reads and clears.
+// (Dev::dt_in_march), also each wave's share of the dt reduction over its
+// tiles (stale maxima of the inactive ones, counts of the active ones and
+// their interior cells), added by atomics into dt slot (wave index) %
+// kDtSlots (one 128-byte line each, so that few waves meet on a line), which
+// the march's dt tail reads and clears.  A wave with nothing to add (no
+// active tile, no stale maxima: most of a sparse domain) adds nothing.
 __global__ __launch_bounds__(256) void k_grow_tiles(Dev d)
 {
-    __shared__ DtRed red4[4];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const bool a = grow_tiles_body<false>(d, t);
     if (!d.dt_in_march) return;   // uniform
-    DtRed r{0.f, 0.f, 0ull, 0};
+    float vx = 0.f, vy = 0.f;
+    unsigned nact = 0u, cells = 0u;
     if (a) {
-        r.cells = tile_cells(d, t);
-        r.nact = 1;
+        cells = (unsigned)tile_cells(d, t);
+        nact = 1u;
     } else if (t < d.ntiles) {
-        r.vx = d.mx[t];
-        r.vy = d.my[t];
+        vx = d.mx[t];
+        vy = d.my[t];
     }
-    r = dt_block_reduce(r, red4);
-    if (threadIdx.x == 0) {
-        unsigned* g = d.gs + 32 * (blockIdx.x % kDtSlots);
-        if (r.vx > 0.f) atomicMax(g, __float_as_uint(r.vx));
-        if (r.vy > 0.f) atomicMax(g + 1, __float_as_uint(r.vy));
-        if (r.nact) {
-            atomicAdd(g + 2, (unsigned)r.nact);
-            atomicAdd(reinterpret_cast<unsigned long long*>(g + 4), r.cells);
+    if (__ballot(nact != 0u || vx > 0.f || vy > 0.f) == 0ull) return;   // uniform
+    for (int off = 32; off > 0; off >>= 1) {
+        vx = fmaxf(vx, __shfl_xor(vx, off));
+        vy = fmaxf(vy, __shfl_xor(vy, off));
+        nact += __shfl_xor(nact, off);
+        cells += __shfl_xor(cells, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsigned* g = d.gs + 32 * ((t >> 6) % kDtSlots);
+        if (vx > 0.f) atomicMax(g, __float_as_uint(vx));
+        if (vy > 0.f) atomicMax(g + 1, __float_as_uint(vy));
+        if (nact) {
+            atomicAdd(g + 2, nact);
+            atomicAdd(reinterpret_cast<unsigned long long*>(g + 4), (unsigned long long)cells);
         }
     }
 }
