@@ -1534,27 +1534,32 @@ int rift_wet_cell_updates(rift_grid* g, long* last_step, long* total)
 struct rift_ensemble {
     std::vector<rift_grid*> members;
     hipStream_t stream = nullptr;
-    Dev* devs = nullptr;            // device copy of the members' Dev
+    Dev* devs = nullptr;            // device copy of the stepped members' Dev
     std::vector<Dev> host_devs;
     unsigned flags = 0;             // TEXTBOOK / MASS_BUDGET (common to all members)
     int march_blocks[2][2] = {{0, 0}, {0, 0}};   // [full|lean][stage]
-    int max_units = 0, max_tiles = 0;
+    int max_units = 0, max_tiles = 0;             // over the stepped members
     int min_rows = kTileH;
     std::vector<Dev> uploaded;      // what devs holds (the graphs read it)
     hipGraphExec_t graph = nullptr; // kEnsGraphSteps captured steps
+    // members seen at their end time by rift_ensemble_finished: no longer
+    // stepped (a step leaves an ended member as it is), until rift_ensemble_set_tf
+    std::vector<char> ended;
+    std::vector<rift_grid*> stepped;   // the members the launches cover
+    double* times = nullptr;        // pinned: the members' times, read together
 };
 
 constexpr int kEnsGraphSteps = 16;
 
 static void ens_step_launches(rift_ensemble* e)
 {
-    const int n = (int)e->members.size();
-    for (rift_grid* g : e->members)
+    const int n = (int)e->stepped.size();
+    for (rift_grid* g : e->stepped)
         if (g->d.flags & (RIFT_RAIN_GRID | RIFT_SURGE_GRID)) launch_track(g->d, e->stream);
     // the lean kernels when no member uses a feature outside them (a member
     // whose flags change changes its Dev, which drops the captured graph)
     bool lean = true;
-    for (rift_grid* g : e->members) lean = lean && march_lean(g->d.flags);
+    for (rift_grid* g : e->stepped) lean = lean && march_lean(g->d.flags);
     const int* blocks = e->march_blocks[lean ? 1 : 0];
     launch_grow2_ens(e->devs, n, e->max_tiles, e->max_units, e->stream);
     launch_march_ens(e->devs, n, e->flags, lean, 1, e->min_rows, blocks[0], e->stream);
@@ -1594,16 +1599,14 @@ int rift_ensemble_create(int n, const rift_desc* descs, rift_ensemble** out)
         if (rc == RIFT_OK) e->members.push_back(g);
     }
     g_member_stream = nullptr;
-    if (rc == RIFT_OK && hipMalloc(&e->devs, sizeof(Dev) * n) != hipSuccess)
-        rc = fail(RIFT_E_HIP, "rift_ensemble_create: device allocation failed");
+    if (rc == RIFT_OK && (hipMalloc(&e->devs, sizeof(Dev) * n) != hipSuccess ||
+                          hipHostMalloc(&e->times, sizeof(double) * n) != hipSuccess))
+        rc = fail(RIFT_E_HIP, "rift_ensemble_create: allocation failed");
     if (rc != RIFT_OK) {
         rift_ensemble_destroy(e);
         return rc;
     }
-    for (rift_grid* g : e->members) {
-        e->max_units = std::max(e->max_units, g->d.nunits);
-        e->max_tiles = std::max(e->max_tiles, g->d.ntiles);
-    }
+    e->ended.assign(n, 0);
     const int cus = e->members[0]->cus;
     for (int lean = 0; lean < 2; ++lean)
         for (int st = 0; st < 2; ++st)
@@ -1626,24 +1629,37 @@ int rift_ensemble_set_tf(rift_ensemble* e, int i, double tf)
     if (!e || i < 0 || i >= (int)e->members.size())
         return fail(RIFT_E_ARG, "rift_ensemble_set_tf: bad argument");
     e->members[i]->d.tf = tf;
+    e->ended[i] = 0;   // stepped again until seen at its (new) end time
     return RIFT_OK;
 }
 
 int rift_ensemble_step(rift_ensemble* e, int nsteps)
 {
     if (!e || nsteps < 0) return fail(RIFT_E_ARG, "rift_ensemble_step: bad argument");
-    const int n = (int)e->members.size();
-    e->host_devs.resize(n);
-    for (int i = 0; i < n; ++i) {
+    // the members not yet seen at their end time: an ended member's step
+    // only applies its pending buffer flips (a read of its fields applies
+    // them too) and clears its per-tile maxima and step counters, so the
+    // launches leave it out and every member's fields, time and tiles are
+    // what stepping it would leave
+    e->stepped.clear();
+    e->host_devs.clear();
+    e->max_units = e->max_tiles = 0;
+    for (size_t i = 0; i < e->members.size(); ++i) {
         rift_grid* g = e->members[i];
         if (!g->inited) return fail(RIFT_E_STATE, "rift_ensemble_step: member not initialised");
         if (g->per_phase) return fail(RIFT_E_UNSUPPORTED, "rift_ensemble_step: member on the per-phase path");
         if (!g->corner_beds)
             return fail(RIFT_E_UNSUPPORTED,
                         "rift_ensemble_step: member's face beds are not means of one corner field");
+        if (e->ended[i]) continue;
         prime(g);
-        e->host_devs[i] = g->d;
+        e->stepped.push_back(g);
+        e->host_devs.push_back(g->d);
+        e->max_units = std::max(e->max_units, g->d.nunits);
+        e->max_tiles = std::max(e->max_tiles, g->d.ntiles);
     }
+    const int n = (int)e->stepped.size();
+    if (n == 0 || nsteps == 0) return RIFT_OK;
     const bool same = e->uploaded.size() == (size_t)n &&
                       std::memcmp(e->uploaded.data(), e->host_devs.data(), sizeof(Dev) * n) == 0;
     if (!same) {
@@ -1662,7 +1678,7 @@ int rift_ensemble_step(rift_ensemble* e, int nsteps)
     for (rift_grid* g : e->members) graphs = graphs && g->use_graphs;
     if (!graphs) {
         for (; left > 0; --left) ens_step_launches(e);
-        for (rift_grid* g : e->members) g->scattered = true;
+        for (rift_grid* g : e->stepped) g->scattered = true;
         return check_launch("rift_ensemble_step");
     }
     if (left >= kEnsGraphSteps && !e->graph) {
@@ -1678,19 +1694,25 @@ int rift_ensemble_step(rift_ensemble* e, int nsteps)
     }
     for (; left >= kEnsGraphSteps; left -= kEnsGraphSteps) HIPCHK(hipGraphLaunch(e->graph, e->stream));
     for (; left > 0; --left) ens_step_launches(e);
-    for (rift_grid* g : e->members) g->scattered = true;
+    for (rift_grid* g : e->stepped) g->scattered = true;
     return check_launch("rift_ensemble_step");
 }
 
 int rift_ensemble_finished(rift_ensemble* e, int* n_finished)
 {
     if (!e || !n_finished) return fail(RIFT_E_ARG, "rift_ensemble_finished: null argument");
+    const int n = (int)e->members.size();
+    for (int i = 0; i < n; ++i)
+        HIPCHK(hipMemcpyAsync(e->times + i, &e->members[i]->d.sc->t, sizeof(double), hipMemcpyDeviceToHost,
+                              e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     int done = 0;
-    for (rift_grid* g : e->members) {
-        double t = 0.0;
-        HIPCHK(hipMemcpyAsync(&t, &g->d.sc->t, sizeof(double), hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
-        if (g->d.tf > 0.0 && t >= g->d.tf) ++done;
+    for (int i = 0; i < n; ++i) {
+        const double tf = e->members[i]->d.tf;
+        if (tf > 0.0 && e->times[i] >= tf) {
+            ++done;
+            e->ended[i] = 1;
+        }
     }
     *n_finished = done;
     return RIFT_OK;
@@ -1721,6 +1743,7 @@ int rift_ensemble_destroy(rift_ensemble* e)
     if (e->graph) (void)hipGraphExecDestroy(e->graph);
     for (rift_grid* g : e->members) rift_destroy(g);
     if (e->devs) (void)hipFree(e->devs);
+    if (e->times) (void)hipHostFree(e->times);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return RIFT_OK;

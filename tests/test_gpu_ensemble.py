@@ -112,6 +112,44 @@ def test_ensemble_end_times(rift):
     e.close()
 
 
+def test_ended_members_leave_the_launches(rift):
+    """Members seen at their end time (rift_ensemble_finished) are no longer
+    launched: the members still running match their single-domain runs bit
+    for bit while the others have left, and each ended member holds the
+    state of its single run at its last step."""
+    members = _members()
+    e = _ensemble(rift, members)
+    singles = [_single(rift, m) for m in members]
+    tfs = [20.0, 1e9, 35.0, 1e9]
+    for i, tf in enumerate(tfs):
+        e.set_tf(i, tf)
+    steps = 0
+    while e.finished() < 2:
+        e.step(4)
+        steps += 4
+        for g, tf in zip(singles, tfs):   # a single run stops where its member does
+            for _ in range(4):
+                if g.time[0] < tf:
+                    g.step(1)
+    assert steps < 400
+    for _ in range(3):   # two members stepped, two left out
+        e.step(9)
+        for k in (1, 3):
+            singles[k].step(9)
+        assert e.finished() == 2
+    for i, (ge, gs) in enumerate(zip(e.members, singles)):
+        assert ge.time == gs.time, (i, ge.time, gs.time)
+        if tfs[i] < 1e9:
+            t, dt = ge.time
+            assert t >= tfs[i] and t - dt < tfs[i], (i, t, dt)
+        for f in ("depth", "hu", "hv", "F"):
+            np.testing.assert_array_equal(ge.field(f), gs.field(f), err_msg=f"{i} {f}")
+        np.testing.assert_array_equal(ge.tiles()[0], gs.tiles()[0])
+    for g in singles:
+        g.close()
+    e.close()
+
+
 def test_ensemble_of_more_than_64_members(rift):
     """Members are located 64 at a time (one per lane); 70 small members of
     varied sizes, some dense and some with few active tiles, still match
