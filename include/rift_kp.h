@@ -327,11 +327,14 @@ int rift_ensemble_destroy(rift_ensemble* e);
 int rift_ensemble_size(rift_ensemble* e);
 rift_grid* rift_ensemble_member(rift_ensemble* e, int i);
 /* End time of member i: from the step whose start time is >= tf on, the
- * member is left as it is (0 = never). */
+ * member is left as it is (0 = never).  Also returns a member that
+ * rift_ensemble_finished saw ended to the launches. */
 int rift_ensemble_set_tf(rift_ensemble* e, int i, double tf);
-/* nsteps batched steps: one step of every member that has not ended. */
+/* nsteps batched steps: one step of every member that has not ended (the
+ * members rift_ensemble_finished saw at their end time are not launched). */
 int rift_ensemble_step(rift_ensemble* e, int nsteps);
-/* Members whose time has reached their end time (blocking). */
+/* Members whose time has reached their end time (blocking: one batch of
+ * reads); they are left out of the following steps' launches. */
 int rift_ensemble_finished(rift_ensemble* e, int* n_finished);
 /* Steps until every member has ended (each needs an end time), checking
  * every check_every steps; max_steps < 0: no limit. */
