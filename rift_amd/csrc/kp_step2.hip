@@ -758,8 +758,30 @@ constexpr bool phased_rows() { return FM == kLeanMask || FM == kCoastMask; }
 // j sits in slot (k + m) mod 3, where k is the phase of row j.  The loop body
 // is instantiated once per phase, so that advancing the window renames
 // registers at compile time instead of copying ~40 values per row.
+// Issue priority of a wave that has marched `done` of the `total` rows of
+// its share: 3 in its first quarter down to 0 in its last.  The SIMD's
+// arbiter prefers the older of two ready waves of equal priority, so with
+// equal shares the waves dispatched first run ahead and the last ones finish
+// alone; a wave that has run ahead yields to those behind it at each
+// quarter of its share.
+__device__ __forceinline__ void pace_wave(int done, int total, int& prio)
+{
+    const int p = (4 * done < total) ? 3 : (2 * done < total) ? 2 : (4 * done < 3 * total) ? 1 : 0;
+    if (p == prio) return;
+    prio = p;
+    switch (p) {
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: __builtin_amdgcn_s_setprio(0); break;
+    }
+}
+
+// sdone / stotal: rows of the wave's share marched before this run and in
+// all (stotal 0: no pacing).
 template <int Stage, bool Textbook, bool Up, bool Budget, unsigned FM>
-__device__ float2 march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane)
+__device__ float2 march_run(const Dev& d, const StepScalars& S, int s, int ja, int jb, int lane, int sdone = 0,
+                            int stotal = 0)
 {
     constexpr int step = Up ? -1 : 1;
     const int c = kStrip * s + lane;
@@ -826,10 +848,13 @@ __device__ float2 march_run(const Dev& d, const StepScalars& S, int s, int ja, i
     float2 wv = make_float2(0.f, 0.f);   // maxima of the tiles flushed (uniform)
     bool wetl = false;
     BudgetAcc acc{0.f, 0.f, 0.f, 0.f};
+    const int jfirst = j;
+    int prio = -1;
     // output row j in phase k; returns whether the run is done
     auto row = [&](auto phase) __attribute__((always_inline)) -> bool {
         constexpr int kc = decltype(phase)::value;
         constexpr int kd = (kc + 1) % 3, ke = (kc + 2) % 3;
+        if (stotal > 0) pace_wave(sdone + (Up ? jfirst - j : j - jfirst), stotal, prio);
         const int rj = (j - 2) / kTileH;   // uniform
         if (rj != rr) {
             flush_tiles<Stage>(d, s, rr, L, out_lane, mxl, myl, wetl, lane, wv);
@@ -1012,8 +1037,8 @@ __device__ __forceinline__ float2 march_body(const Dev& d)
         const int ja = 2 + (int)((long long)rows * mi / m);
         const int jb = 2 + (int)((long long)rows * (mi + 1) / m);
         if (ja >= jb) return wv;
-        if (mi & 1) keep(march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane));
-        else keep(march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane));
+        if (mi & 1) keep(march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane, 0, jb - ja));
+        else keep(march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane, 0, jb - ja));
         return wv;
     }
 
@@ -1058,7 +1083,9 @@ __device__ __forceinline__ float2 march_body(const Dev& d)
         const int jb = min(rlast * kTileH + 2 + (int)(pe - (long long)kk * kTileH), d.ny - 2);
         // a quad's wave w marches strip 4c + w
         const int s = quad ? 4 * c + wid : c;
-        if (ja < jb) keep(march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane));
+        const int sdone = (int)((long long)k * kTileH + a - p0);
+        if (ja < jb)
+            keep(march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane, sdone, (int)(pend - p0)));
         k = kk + 1;
         a = 0;
         if ((long long)k * kTileH >= pend) break;
