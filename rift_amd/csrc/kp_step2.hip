@@ -1748,12 +1748,14 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
             S = step_scalars(d);
             smem = mem;
         }
+        // pacing: the wave's shares count B rows each
+        const int stotal = (int)(i1 - i0) * B, sdone = (int)(i - i0) * B;
         if (mcnt == d.nunits) {
             const int band = ls / d.nstrips, s = ls - band * d.nstrips;
             const int ja = 2 + band * B, jb = min(ja + B, d.ny - 2);
             if (ja >= jb) continue;
-            if (band & 1) march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane);
-            else march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
+            if (band & 1) march_run<Stage, Textbook, true, Budget, FM>(d, S, s, ja, jb, lane, sdone, stotal);
+            else march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane, sdone, stotal);
         } else {
             // units [ls bu, (ls + 1) bu) of the list: runs of vertically
             // adjacent units of one strip
@@ -1761,6 +1763,7 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
             int kk = ls * bu;
             const int kend = min(kk + bu, mcnt);
             while (kk < kend) {
+                const int kdone = (kk - ls * bu) * kTileH;
                 const int u = __builtin_amdgcn_readfirstlane(d.ulist[kk]);
                 const int s = u / d.nurows, rr = u - s * d.nurows;
                 int last = u;
@@ -1772,7 +1775,8 @@ __global__ __launch_bounds__(256, Waves) void k_march_ens(const Dev* __restrict_
                 }
                 const int ja = rr * kTileH + 2;
                 const int jb = min((last - s * d.nurows + 1) * kTileH + 2, d.ny - 2);
-                if (ja < jb) march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane);
+                if (ja < jb)
+                    march_run<Stage, Textbook, false, Budget, FM>(d, S, s, ja, jb, lane, sdone + kdone, stotal);
                 ++kk;
             }
         }
