@@ -213,9 +213,10 @@ int rift_set_option(rift_grid* g, int option, int value);
 /* Autotuner (no counterpart in the reference, whose 16 x 14 launch shape is
  * fixed, grid.cu:18-19): times every launch variant of the fused step on a
  * private copy of the handle's current state (nsteps graph-replayed steps
- * each, after two warm-up steps), selects the fastest on the handle and
- * returns it in *best; ms_per_step (optional, RIFT_NVARIANTS entries)
- * receives each variant's time.  The handle's own state is not advanced.
+ * each, after two warm-up steps), times the six fastest twice more, selects
+ * the one with the least total on the handle and returns it in *best;
+ * ms_per_step (optional, RIFT_NVARIANTS entries) receives each variant's
+ * time in the first round.  The handle's own state is not advanced.
  * Needs device memory for a second copy of the domain. */
 int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step);
 

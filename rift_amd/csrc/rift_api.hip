@@ -979,34 +979,53 @@ int rift_autotune(rift_grid* g, int nsteps, int* best, double* ms_per_step)
         return rc;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    double t_best = 0.0;
-    int v_best = -1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
         rc = fail(RIFT_E_HIP, "rift_autotune: hipEventCreate failed");
     }
-    for (int v = 0; v < RIFT_NVARIANTS && rc == RIFT_OK; ++v) {
-        // warm-up from the snapshot (captures the variant's graphs), then
-        // the timed run from the same snapshot
+    // one variant timed from the snapshot: a warm-up run (captures the
+    // variant's graphs), then the timed run from the same snapshot
+    auto time_variant = [&](int v, double& per) -> int {
         c->per_phase = g->per_phase;
         c->use_graphs = g->use_graphs;   // timed the way the handle steps
         c->dt_in_march = g->dt_in_march;
-        if ((rc = copy_state(g, c)) || (rc = rift_set_option(c, RIFT_OPT_VARIANT, v))) break;
-        if ((rc = rift_step(c, nsteps)) || (rc = copy_state(g, c))) break;
-        if ((rc = rift_step(c, 2))) break;
-        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(RIFT_E_HIP, "hipEventRecord"); break; }
-        if ((rc = rift_step(c, nsteps))) break;
+        int r;
+        if ((r = copy_state(g, c)) || (r = rift_set_option(c, RIFT_OPT_VARIANT, v))) return r;
+        if ((r = rift_step(c, nsteps)) || (r = copy_state(g, c))) return r;
+        if ((r = rift_step(c, 2))) return r;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) return fail(RIFT_E_HIP, "hipEventRecord");
+        if ((r = rift_step(c, nsteps))) return r;
         float ms = 0.f;
         if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = fail(RIFT_E_HIP, "rift_autotune: event timing failed");
-            break;
-        }
-        const double per = (double)ms / nsteps;
-        if (ms_per_step) ms_per_step[v] = per;
-        if (v_best < 0 || per < t_best) {
-            t_best = per;
-            v_best = v;
-        }
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return fail(RIFT_E_HIP, "rift_autotune: event timing failed");
+        per = (double)ms / nsteps;
+        return RIFT_OK;
+    };
+    std::vector<double> t(RIFT_NVARIANTS, 0.0);
+    for (int v = 0; v < RIFT_NVARIANTS && rc == RIFT_OK; ++v) rc = time_variant(v, t[v]);
+    int v_best = -1;
+    if (rc == RIFT_OK) {
+        if (ms_per_step)
+            for (int v = 0; v < RIFT_NVARIANTS; ++v) ms_per_step[v] = t[v];
+        // one timing of a variant varies by about 1 %, as much as the
+        // fastest variants differ: the fastest few are timed twice more, in
+        // turn, and the one with the least total is kept
+        constexpr int kTop = 6;
+        std::vector<int> order(RIFT_NVARIANTS);
+        for (int v = 0; v < RIFT_NVARIANTS; ++v) order[v] = v;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t[a] < t[b]; });
+        std::vector<double> total(kTop);
+        for (int k = 0; k < kTop; ++k) total[k] = t[order[k]];
+        for (int rep = 0; rep < 2 && rc == RIFT_OK; ++rep)
+            for (int k = 0; k < kTop && rc == RIFT_OK; ++k) {
+                double per = 0.0;
+                rc = time_variant(order[k], per);
+                total[k] += per;
+            }
+        int kb = 0;
+        for (int k = 1; k < kTop; ++k)
+            if (total[k] < total[kb]) kb = k;
+        v_best = order[kb];
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

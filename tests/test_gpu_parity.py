@@ -322,7 +322,8 @@ def test_launch_variants_bit_identical_and_autotuner(rift):
     g.step(20)
     before = (g.time, g.field("depth"))
     best, ms = g.autotune(16)
-    assert 0 <= best < rift.NVARIANTS and (ms > 0).all() and ms[best] == ms.min()
+    # the fastest six of the sweep are timed twice more: the kept one is among them
+    assert 0 <= best < rift.NVARIANTS and (ms > 0).all() and best in np.argsort(ms, kind="stable")[:6]
     assert g.time == before[0]
     np.testing.assert_array_equal(g.field("depth"), before[1])
     g.step(20)   # continues with the chosen variant
