@@ -100,15 +100,31 @@ void drop_graphs(rift_grid* g)
 
 namespace {
 
-float* alloc_plane(rift_grid* g)
+// Cell fields start kPlaneShift floats into their allocation (which has
+// kPlaneSlack floats more than the fields need): column 2, the first
+// interior column and tile column 0's first, then falls on a 64-byte
+// boundary in every row (the pitch is a multiple of 64 floats), and each
+// tile's 16 columns of a row are exactly one 64-byte sector.  Neighbouring
+// tiles keep their state in different buffers whenever they became active
+// at steps of different parity, so with tiles straddling sectors (column 2
+// at byte 8) every sector of a row was written half by one tile and half by
+// the other in another plane: stage 2 wrote 1.56x the bytes of U^{n+1}.
+constexpr size_t kPlaneShift = 14, kPlaneSlack = 16;
+
+// n floats of cell fields (one or more consecutive planes), zeroed; returns
+// the shifted start.
+float* alloc_cells(rift_grid* g, size_t n)
 {
     void* p = nullptr;
-    if (hipMalloc(&p, g->plane * sizeof(float)) != hipSuccess) return nullptr;
-    if (hipMemset(p, 0, g->plane * sizeof(float)) != hipSuccess) return nullptr;
+    const size_t bytes = (n + kPlaneSlack) * sizeof(float);
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    if (hipMemset(p, 0, bytes) != hipSuccess) return nullptr;
     g->allocs.push_back(p);
-    g->alloc_sizes.push_back(g->plane * sizeof(float));
-    return static_cast<float*>(p);
+    g->alloc_sizes.push_back(bytes);
+    return static_cast<float*>(p) + kPlaneShift;
 }
+
+float* alloc_plane(rift_grid* g) { return alloc_cells(g, g->plane); }
 
 template <class T>
 T* alloc_bytes(rift_grid* g, size_t n)
@@ -357,13 +373,9 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     // select is one plane (d.soff elements) added to the h/hu/hv indices and
     // fits the 32-bit lane offsets of the marching kernels
     {
-        void* p = nullptr;
-        ok = hipMalloc(&p, 6 * g->plane * sizeof(float)) == hipSuccess &&
-             hipMemset(p, 0, 6 * g->plane * sizeof(float)) == hipSuccess;
+        float* b = alloc_cells(g, 6 * g->plane);
+        ok = b != nullptr;
         if (ok) {
-            g->allocs.push_back(p);
-            g->alloc_sizes.push_back(6 * g->plane * sizeof(float));
-            float* b = static_cast<float*>(p);
             d.h = b; d.h0 = b + g->plane;
             d.hu = b + 2 * g->plane; d.hu0 = b + 3 * g->plane;
             d.hv = b + 4 * g->plane; d.hv0 = b + 5 * g->plane;
@@ -373,7 +385,7 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     P(d.dh); P(d.dhu); P(d.dhv); P(d.G); P(d.n); P(d.BC); P(d.BX); P(d.BY);
     // corners: one row more than a plane's padding needs is never read;
     // the marching kernels' last lane reads one column past its own
-    d.DC = alloc_bytes<float>(g, g->plane + (size_t)d.P);
+    d.DC = alloc_cells(g, g->plane + (size_t)d.P);
     ok = ok && d.DC;
     if (f & RIFT_INFILTRATION) { P(d.F); P(d.F0); P(d.dF); P(d.K); }
     if (f & RIFT_SAVE_MAX) { P(d.hmax); P(d.qmax); P(d.tpeak); P(d.tdry); }
@@ -1045,13 +1057,8 @@ int rift_map_series_create(rift_grid* g, int which, int nslots, int deltat, doub
     if (g->d.sg_series) return fail(RIFT_E_STATE, "rift_map_series_create: series exists");
     if (nslots < 2 || deltat <= 0) return fail(RIFT_E_ARG, "rift_map_series_create: nslots >= 2, deltat > 0");
     Dev& d = g->d;
-    void* p = nullptr;
-    const size_t bytes = (size_t)nslots * g->plane * sizeof(float);
-    HIPCHK(hipMalloc(&p, bytes));
-    g->allocs.push_back(p);
-    g->alloc_sizes.push_back(bytes);
-    HIPCHK(hipMemsetAsync(p, 0, bytes, g->stream));
-    d.sgs = static_cast<float*>(p);
+    d.sgs = alloc_cells(g, (size_t)nslots * g->plane);
+    if (!d.sgs) return fail(RIFT_E_HIP, "rift_map_series_create: device allocation failed");
     d.sgflag = alloc_bytes<unsigned char>(g, (size_t)nslots * d.ntiles);
     d.sgtag = alloc_bytes<int>(g, nslots);
     d.sgu = alloc_bytes<unsigned char>(g, d.nunits);
