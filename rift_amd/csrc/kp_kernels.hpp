@@ -80,13 +80,7 @@ struct Scalars {
     int ustep;
 };
 
-// One workgroup's share of the fused step's dt reduction.
-struct DtPartial {
-    float vx, vy;
-    int nact, pad;
-    unsigned long long cells;
-};
-constexpr int kDtMaxBlocks = 512;   // workgroups of the dt kernel (partials per handle)
+constexpr int kDtMaxBlocks = 512;   // workgroups of the dt kernel
 // Slots of the dt reduction formed in the stage-1 march (128 bytes apart),
 // which grow and the march add into.
 constexpr int kDtSlots = 256;
@@ -128,7 +122,7 @@ struct Dev {
     float *mx, *my;              // per-tile maxima of the last flux pass
     float *mx1, *my1;            // fused step: stage-1 maxima of this step
     Scalars* sc;
-    DtPartial* dtp;              // kDtMaxBlocks partials of the fused dt kernel
+    unsigned* dtk;               // dt reductions: 8 per-XCD ticket counters, 128 bytes apart
     // single domain, dt formed by the stage-1 march: kDtSlots slots of 32
     // words (vx, vy bits, active tiles, -, cells as 64 bits)
     unsigned* gs;

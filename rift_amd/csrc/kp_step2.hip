@@ -1143,16 +1143,18 @@ __global__ __launch_bounds__(256) void k_clear_sel(Dev d)
 
 // dt of the fused step: max over every tile of the stage-1 maxima (active
 // tiles) or the stage-2 maxima of the last step a tile was active (others),
-// then ComputeTimestep's arithmetic.  The same pass counts the active tiles
-// and their interior cells.  Each workgroup reduces kDtTilesPerBlock
-// consecutive tiles (four independent loads per lane in flight) into its
-// slot of the handle's partials array; the last workgroup to finish (ticket)
-// reduces the partials in a fixed order, forms dt, moves the unit count to
-// stage 2 and evaluates the next step's forcing at the new time
-// (UpdateSource runs at the start of the next RunSimulation,
-// simulator.cu:780-784, with the t of this step).  No same-address atomics
-// beyond the ticket: with one atomic per workgroup and quantity, the 300k
-// tiles of an 8192^2 domain took 41 us over 32 workgroups.
+// then ComputeTimestep's arithmetic.  The same reduction counts the active
+// tiles and their interior cells.  The dt kernel (RIFT_OPT_DT_KERNEL 1, and
+// ensembles) reduces kDtTilesPerBlock consecutive tiles per workgroup (four
+// independent loads per lane in flight) into a dt slot (dt_slot_add); by
+// default grow and the stage-1 march add into the slots instead
+// (march1_dt).  The last workgroup to finish (ticket) reduces the slots,
+// forms dt, moves the unit count to stage 2 and evaluates the next step's
+// forcing at the new time (UpdateSource runs at the start of the next
+// RunSimulation, simulator.cu:780-784, with the t of this step).  Few
+// workgroups meet on one address: with one atomic per workgroup and
+// quantity on a single line, the 300k tiles of an 8192^2 domain took 41 us
+// over 32 workgroups.
 constexpr int kDtTilesPerBlock = 1024;
 
 __host__ __device__ inline int dt_blocks(int ntiles)
@@ -1191,11 +1193,10 @@ __device__ __forceinline__ DtRed dt_block_reduce(DtRed r, DtRed* lds4)
     return r;
 }
 
-// Per-XCD ticket x of the dt kernel: a counter on its own 128-byte line
-// after the partial records.
+// Per-XCD ticket x of the dt reductions: a counter on its own 128-byte line.
 __device__ __forceinline__ unsigned* dt_xticket(const Dev& d, int x)
 {
-    return reinterpret_cast<unsigned*>(d.dtp + kDtMaxBlocks) + 32 * x;
+    return d.dtk + 32 * x;
 }
 
 // The dt reductions of the fused step (the dt kernel's, or the stage-1

@@ -407,13 +407,13 @@ int rift_create(const rift_desc* desc, rift_grid** out)
     d.qlist = alloc_bytes<int>(g, (size_t)d.nquads * d.nurows);
     d.utag = alloc_bytes<int>(g, d.nunits);
     d.sc = alloc_bytes<Scalars>(g, 1);
-    // the partial records, then 8 ticket counters a 128-byte line apart
-    d.dtp = alloc_bytes<DtPartial>(g, kDtMaxBlocks + (8 * 128) / sizeof(DtPartial) + 1);
+    // 8 ticket counters a 128-byte line apart
+    d.dtk = alloc_bytes<unsigned>(g, 8 * 32);
     // dt formed by the stage-1 march: the slots grow and the march add into
     d.gs = alloc_bytes<unsigned>(g, 32 * (size_t)kDtSlots);
     d.ser = alloc_bytes<SeriesTable>(g, 3);
     g->dsum = alloc_bytes<double>(g, 4);
-    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.dtp && d.gs && d.ser &&
+    ok = ok && d.wet && d.act && d.list && d.mx && d.my && d.mx1 && d.my1 && d.sc && d.dtk && d.gs && d.ser &&
          g->dsum && d.sel &&
          d.flip && d.ulist && d.umark && d.qlist && d.utag;
     if (!ok) {
