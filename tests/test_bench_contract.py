@@ -43,3 +43,24 @@ def test_stage_bytes_add_up_to_the_step():
     import bench
     for w in bench.STAGE_BYTES.values():
         assert w["march1"] + w["march2"] == w["step"]
+
+
+@pytest.mark.parametrize("which,variants", [
+    ("COASTAL_PROFILE", [0, 1, 4, 5, 8, 9, 12, 13]),
+    ("RAIN_PROFILE", [0, 1, 2, 3, 8, 9, 10, 11, 32, 33, 34, 35, 40, 41, 42, 43])])
+def test_committed_profiles_cover_the_variants_the_autotuner_picks(which, variants):
+    """The launch variants the autotuner has picked on these workloads (the
+    classes within about 1 % of the fastest) have a measured stage-2 traffic
+    line in the committed profile, recorded on the profile's own state, of a
+    plausible size: between the algorithmic bytes of the launch and 1.6x
+    them."""
+    sys.path.insert(0, ROOT)
+    import bench
+    path = getattr(bench, which)
+    states = [ln.strip()[len("state: "):] for ln in open(path) if ln.startswith("state: ")]
+    assert len(states) == 1
+    algo = {"COASTAL_PROFILE": 6621024 * 44, "RAIN_PROFILE": 4194304 * 56}[which]
+    for v in variants:
+        t = bench.measured_traffic(path, "k_march<2>", states[0], v)
+        assert t is not None, (which, v)
+        assert algo <= t <= 1.6 * algo, (which, v, t / algo)
