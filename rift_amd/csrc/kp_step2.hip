@@ -747,17 +747,6 @@ struct Phase {
 template <unsigned FM>
 constexpr bool phased_rows() { return FM == kLeanMask || FM == kCoastMask; }
 
-// One run of a wave: output rows [ja, jb) of strip s, marched downwards
-// (increasing j) or upwards.  The window holds the output row C, the next
-// row D (in marching direction) and the raw row after it; the y flux of the
-// face behind C is carried from the previous row.  Crossing into another
-// tile row flushes the per-tile results and shifts the lane's tile-row
-// window.
-//
-// The window lives in three slots per quantity: row j + m*step of output row
-// j sits in slot (k + m) mod 3, where k is the phase of row j.  The loop body
-// is instantiated once per phase, so that advancing the window renames
-// registers at compile time instead of copying ~40 values per row.
 // Issue priority of a wave that has marched `done` of the `total` rows of
 // its share: 3 at its start down to 0 near its end.  The SIMD's arbiter
 // prefers the older of two ready waves of equal priority, so with equal
@@ -781,6 +770,18 @@ __device__ __forceinline__ void pace_wave(int done, int total, int& prio, bool d
     }
 }
 
+// One run of a wave: output rows [ja, jb) of strip s, marched downwards
+// (increasing j) or upwards.  The window holds the output row C, the next
+// row D (in marching direction) and the raw row after it; the y flux of the
+// face behind C is carried from the previous row.  Crossing into another
+// tile row flushes the per-tile results and shifts the lane's tile-row
+// window.
+//
+// The window lives in three slots per quantity: row j + m*step of output row
+// j sits in slot (k + m) mod 3, where k is the phase of row j.  The loop body
+// is instantiated once per phase, so that advancing the window renames
+// registers at compile time instead of copying ~40 values per row.
+//
 // sdone / stotal: rows of the wave's share marched before this run and in
 // all (stotal 0: no pacing); dense: the share is one band of a fully active
 // domain (pace_wave's schedule).
