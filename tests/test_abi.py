@@ -74,3 +74,18 @@ def test_kappa_matches_oracle():
     import oracle_lib as O
     for dx in (1.0, 5.0, 10.0, 0.3):
         assert kp.kappa(dx, dx) == O.lib().orc_kappa(dx, dx)
+
+
+def test_binding_constants_match_the_header():
+    """Every flag, option and count rift_kp.h defines has the same value in
+    the Python binding (a flag bit or option number out of step would select
+    another feature without any error)."""
+    text = open(os.path.join(ROOT, "include", "rift_kp.h")).read()
+    defs = {}
+    for m in re.finditer(r"^#define RIFT_(\w+)\s+\((?:1u\s*<<\s*(\d+))\)", text, re.M):
+        defs[m.group(1)] = 1 << int(m.group(2))
+    for m in re.finditer(r"^#define RIFT_(OPT_\w+|NVARIANTS)\s+(\d+)", text, re.M):
+        defs[m.group(1)] = int(m.group(2))
+    assert {"TILE_ACCEL", "MASS_BUDGET", "OPT_DT_KERNEL", "NVARIANTS"} <= set(defs)
+    for name, value in defs.items():
+        assert getattr(kp, name) == value, name
